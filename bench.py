@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Benchmark of the Godunov step (HLLC, perfect gas) at 512^3 on one MI355X.
+
+A "step" is one godunov update (godunov.hpp:15-92) over the 512^3 interior of
+514^3 arrays, FP64, int64 indexing, on a seeded random state (rho ~ U[0.5,2],
+u_k ~ N(0,0.3), p ~ U[0.5,2], gamma = 1.4, dx = 1/512; dt = 0.5 * time_step).
+`value` is interior cell-updates per second of that step with all inputs resident
+in HBM. The JSON line also carries the roofline of the dominant kernel (the tiled
+godunov kernel, algorithmic bytes s*(5*N_arr + 10*N_int) per launch), the CPU
+oracle timed on the host beside it, and secondary figures (fused full step,
+conversions, time step, FP32, reference-order kernel) under "extra".
+
+    python bench.py [--steps K] [--warmup W] [--n 512] [--no-cpu] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s & % HBM peak, 512^3 Godunov step; batched & adjoint vs plain"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512, help="interior cells per axis")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
+    ap.add_argument("--quick", action="store_true", help="headline godunov only (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 4 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--tile", type=str, default=None, help="godunov tile 'by,kchunk'")
+    return ap.parse_args()
+
+
+class Timer:
+    """HIP events on the stream the kernels are launched on."""
+
+    def __init__(self, torch, stream):
+        self.torch = torch
+        self.stream = stream
+
+    def run(self, fn, reps: int) -> float:
+        torch = self.torch
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(self.stream)
+        for _ in range(reps):
+            fn()
+        end.record(self.stream)
+        end.synchronize()
+        return start.elapsed_time(end) / reps  # ms per launch
+
+
+def cpu_baseline(n: int, budget_s: float = 10.0):
+    """The CPU oracle (reference-order restatement, OpenMP) timed on a bounded slab
+    of the same workload: full 514 x 514 planes, as many as fit the budget."""
+    import numpy as np
+
+    from oracle import Oracle
+
+    orc = Oracle()
+    threads = orc.threads()
+    gamma = 1.4
+    dx = 1.0 / n
+
+    def sample(nplanes):
+        rng = np.random.default_rng(1234)
+        shape = (5, nplanes + 2, n + 2, n + 2)
+        prim = np.empty(shape)
+        prim[0] = rng.uniform(0.5, 2.0, shape[1:])
+        prim[1] = rng.uniform(0.5, 2.0, shape[1:])
+        prim[2:] = rng.normal(0.0, 0.3, (3,) + shape[1:])
+        cons = orc.prim_to_cons(prim, gamma)
+        dt = 0.5 * orc.time_step(prim, gamma, dx, dx, dx)
+        return prim, cons, dt
+
+    prim, cons, dt = sample(2)
+    t0 = time.perf_counter()
+    orc.godunov(prim, cons, gamma, dx, dx, dx, dt)
+    per_plane = (time.perf_counter() - t0) / 2
+    nplanes = int(max(2, min(n, budget_s / 3 / max(per_plane, 1e-9))))
+    prim, cons, dt = sample(nplanes)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.godunov(prim, cons, gamma, dx, dx, dx, dt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    cells = n * n * nplanes
+    return {
+        "value": cells / t / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"godunov FP64 on {nplanes} interior planes of the {n}^3 workload "
+                  f"({n}x{n}x{nplanes} cells, 514-wide arrays), median of 3, {threads} OpenMP threads, "
+                  f"oracle = reference-order C++ restatement (Kokkos unavailable offline)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the multi-GPU halo design is documented, not built)"}))
+        sys.exit(2)
+
+    import torch
+
+    from __graft_entry__ import load_package
+
+    eb = load_package()
+    if args.tile:
+        by, kc = (int(v) for v in args.tile.split(","))
+        eb.set_godunov_tile(by, kc)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ex = eb.ExecSpace(dev)
+    timer = Timer(torch, ex.stream)
+    n = args.n
+    na = n + 2
+    n_int = n**3
+    n_arr = na**3
+    gamma = 1.4
+    eos = eb.PerfectGas(gamma)
+    mesh = eb.UniformMesh3d(1.0 / n, 1.0 / n, 1.0 / n)
+
+    def alloc(dtype):
+        prim = eb.to_mdspan(eb.create_prim_arrays_1d(ex, n_arr, dtype), na, na, na, index_bits=64)
+        cons = eb.to_mdspan(eb.create_cons_arrays_1d(ex, n_arr, dtype), na, na, na, index_bits=64)
+        eb.init_random(ex, prim, seed=1234)
+        eb.prim_to_cons(ex, prim, cons, eos)
+        return prim, cons
+
+    log(f"allocating {n}^3 FP64 state")
+    prim, cons = alloc(torch.float64)
+    dt = 0.5 * eb.time_step(ex, prim, eos, mesh)
+    hllc = eb.hllc()
+
+    def step():
+        eb.godunov(ex, prim, cons, eos, mesh, hllc, dt)
+
+    log(f"warmup {args.warmup}")
+    timer.run(step, max(args.warmup, 1))
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} godunov steps")
+    t0 = time.perf_counter()
+    ms = timer.run(step, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_wall = wall * 1e3 / args.steps
+    s = 8
+    bytes_godunov = s * (5 * n_arr + 10 * n_int)
+    achieved = bytes_godunov / (ms * 1e-3) / 1e9
+    value = n_int / (ms_wall * 1e-3) / 1e6
+    log(f"godunov FP64: {ms:.3f} ms/launch (events), {ms_wall:.3f} ms/step (wall), {achieved:.0f} GB/s")
+
+    extra = {
+        "godunov_f64_ms_event": round(ms, 4),
+        "godunov_f64_Mcells_per_s_ref_convention": round(n_arr / (ms * 1e-3) / 1e6, 1),
+        "tile": list(eb.get_godunov_tile()),
+    }
+
+    if not args.quick:
+        reps = max(3, args.steps // 2)
+        # Reference-order kernel (six solves per cell), same state.
+        ms_ro = timer.run(lambda: eb.godunov(ex, prim, cons, eos, mesh, hllc, dt, reference_order=True), reps)
+        extra["godunov_reference_order_f64_ms"] = round(ms_ro, 4)
+        # Pointwise operators and the reduction (n^3 conventions of the reference
+        # benchmarks use (n+2)^3 arrays here; bytes counted on the arrays timed).
+        ms_c2p = timer.run(lambda: eb.cons_to_prim(ex, cons, prim, eos), reps)
+        ms_p2c = timer.run(lambda: eb.prim_to_cons(ex, prim, cons, eos), reps)
+        dt_dev = torch.empty(1, dtype=torch.float64, device=dev)
+        ms_ts = timer.run(lambda: eb.time_step_async(ex, prim, eos, mesh, out=dt_dev), reps)
+        extra["cons_to_prim_f64"] = {"ms": round(ms_c2p, 4), "GBps": round(10 * s * n_arr / ms_c2p / 1e6, 1)}
+        extra["prim_to_cons_f64"] = {"ms": round(ms_p2c, 4), "GBps": round(10 * s * n_arr / ms_p2c / 1e6, 1)}
+        extra["time_step_f64"] = {"ms": round(ms_ts, 4), "GBps": round(5 * s * n_arr / ms_ts / 1e6, 1)}
+        # Fused full step (godunov + cons_to_prim + next dt, dt on device).
+        sim = eb.simulation.Simulation(ex, prim, cons, eos, mesh, 0.5)
+        sim.run(4, use_graph=True)  # eager first pair, then graph capture + one replay
+        ms_fused = timer.run(lambda: sim.run(2, use_graph=True), max(2, reps // 2)) / 2
+        fused_bytes = s * (5 * n_arr + 15 * n_int)
+        extra["step_fused_f64"] = {
+            "ms": round(ms_fused, 4),
+            "Mcell_updates_per_s": round(n_int / ms_fused / 1e3, 1),
+            "GBps_algorithmic": round(fused_bytes / ms_fused / 1e6, 1),
+            "hbm_frac": round(fused_bytes / ms_fused / 1e6 / HBM_PEAK_GBS, 4),
+        }
+        # Reference-order full step: time_step (sync) + godunov + cons_to_prim.
+        def ref_step():
+            d = eb.time_step(ex, prim, eos, mesh)
+            eb.godunov(ex, prim, cons, eos, mesh, hllc, 0.5 * d)
+            eb.cons_to_prim(ex, cons, prim, eos)
+        ms_refstep = timer.run(ref_step, max(2, reps // 2))
+        extra["step_reference_order_f64_ms"] = round(ms_refstep, 4)
+        del sim
+        log("FP32 state")
+        del prim, cons
+        torch.cuda.empty_cache()
+        prim32, cons32 = alloc(torch.float32)
+        dt32 = 0.5 * eb.time_step(ex, prim32, eos, mesh)
+        timer.run(lambda: eb.godunov(ex, prim32, cons32, eos, mesh, hllc, dt32), 2)
+        ms32 = timer.run(lambda: eb.godunov(ex, prim32, cons32, eos, mesh, hllc, dt32), args.steps)
+        extra["godunov_f32"] = {
+            "ms": round(ms32, 4),
+            "Mcell_updates_per_s": round(n_int / ms32 / 1e3, 1),
+            "GBps_algorithmic": round(4 * (5 * n_arr + 10 * n_int) / ms32 / 1e6, 1),
+            "speedup_vs_f64": round(ms / ms32, 3),
+        }
+        del prim32, cons32
+        torch.cuda.empty_cache()
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 29, dtype=torch.float64, device=dev)  # 4 GiB
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        ms_copy = timer.run(lambda: dst.copy_(src), 5)
+        extra["copy_calibration"] = {"bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8,
+                                     "ms": round(ms_copy, 4), "GBps": round(2 * src.numel() * 8 / ms_copy / 1e6, 1)}
+        del src, dst
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle baseline")
+        cpu = cpu_baseline(n)
+        log(f"CPU: {cpu['value']:.2f} Mcell-updates/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"godunov step (HLLC, perfect gas), {n}^3 interior cells in {na}^3 arrays, FP64, "
+                        f"int64 indexing, seeded random state rho~U[0.5,2] u~N(0,0.3) p~U[0.5,2] seed 1234, "
+                        f"gamma 1.4, dt = 0.5*time_step",
+            "interior_cells": n_int,
+            "array_cells": n_arr,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "eb::k_godunov_tiled<double, long, %d, false>" % eb.get_godunov_tile()[0],
+            "bytes_per_launch": bytes_godunov,
+            "avg_launch_ms": round(ms, 4),
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    ex.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,113 @@
+"""ctypes binding of libeuler_hip.so (the C ABI declared in include/euler_capi.h).
+
+The library is loaded from this package's directory. There is no fallback: if the
+shared object is missing, importing this module raises, so nothing can silently run
+a non-HIP path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libeuler_hip.so"
+
+EB_OK = 0
+EB_ERR_INVALID_ARGUMENT = 1
+EB_ERR_INDEX_OVERFLOW = 2
+EB_ERR_HIP = 3
+EB_ERR_OUT_OF_MEMORY = 4
+EB_ERR_UNSUPPORTED = 5
+
+
+class EulerError(RuntimeError):
+    """Raised when an eb_* entry point returns a non-zero status."""
+
+    def __init__(self, status: int, func: str, message: str):
+        super().__init__(f"{func}: status {status}: {message}")
+        self.status = status
+
+
+def _fields(names):
+    return [(n, ctypes.c_void_p) for n in names]
+
+
+class PrimPtrs(ctypes.Structure):
+    """eb_prim_f64 / eb_prim_f32: {d, p, ux0, ux1, ux2} device pointers."""
+
+    _fields_ = _fields(["d", "p", "ux0", "ux1", "ux2"])
+
+
+class ConsPtrs(ctypes.Structure):
+    """eb_cons_f64 / eb_cons_f32: {d, e, mx0, mx1, mx2} device pointers."""
+
+    _fields_ = _fields(["d", "e", "mx0", "mx1", "mx2"])
+
+
+_P = ctypes.POINTER
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_u64 = ctypes.c_uint64
+_f64 = ctypes.c_double
+_f32 = ctypes.c_float
+
+
+def _signatures():
+    sig = {
+        "eb_context_create": [_P(_vp)],
+        "eb_context_destroy": [_vp],
+        "eb_set_godunov_tile": [_int, _int],
+        "eb_get_godunov_tile": [_P(_int), _P(_int)],
+    }
+    for suf, real in (("f64", _f64), ("f32", _f32)):
+        ext = [_i64, _i64, _i64, _int]
+        sig[f"eb_cons_to_prim_{suf}"] = [_vp, _P(ConsPtrs), _P(PrimPtrs)] + ext + [real]
+        sig[f"eb_prim_to_cons_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs)] + ext + [real]
+        sig[f"eb_time_step_{suf}"] = [_vp, _vp, _P(PrimPtrs)] + ext + [real] * 4 + [_vp]
+        sig[f"eb_time_step_sync_{suf}"] = [_vp, _vp, _P(PrimPtrs)] + ext + [real] * 4 + [_P(real)]
+        sig[f"eb_godunov_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs)] + ext + [real] * 5
+        sig[f"eb_godunov_dev_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs)] + ext + [real] * 4 + [_vp, real]
+        sig[f"eb_godunov_reference_order_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs)] + ext + [real] * 5
+        sig[f"eb_ghost_refresh_{suf}"] = [_vp, _vp, _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 4 + [_vp]
+        sig[f"eb_step_fused_{suf}"] = (
+            [_vp, _vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 5 + [_vp, _vp, _vp, _vp]
+        )
+        sig[f"eb_init_implode_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 3
+        sig[f"eb_init_diagonal_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 8
+        sig[f"eb_init_random_{suf}"] = [_vp, _P(PrimPtrs), _i64, _int, _u64] + [_f64] * 5
+    return sig
+
+
+SIGNATURES = _signatures()
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C {LIB_PATH.parent}` "
+            "(or __graft_entry__.build()); there is no non-HIP fallback"
+        )
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.eb_status_string.argtypes = [ctypes.c_int]
+    lib.eb_status_string.restype = ctypes.c_char_p
+    lib.eb_last_error.argtypes = []
+    lib.eb_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int, func: str) -> None:
+    if status != EB_OK:
+        msg = lib.eb_last_error().decode(errors="replace")
+        raise EulerError(status, func, msg or lib.eb_status_string(status).decode())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib, name)(*args), name)

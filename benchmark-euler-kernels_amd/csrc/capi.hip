@@ -1,0 +1,656 @@
+// extern "C" layer of the Godunov-step library: argument checking, index-type
+// dispatch (the reference's index_t switch), launch shapes, status codes.
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <string>
+
+#include <hip/hip_runtime.h>
+
+#include "euler_capi.h"
+#include "kernels.hpp"
+
+struct eb_context
+{
+    int device;
+    eb::MinScratch* scratch; // one slot: reductions through one context are serialised
+    double* dt_slot;         // device landing slot for the blocking time step
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+std::atomic<int> g_tile_by {8};
+std::atomic<int> g_tile_kchunk {64};
+
+int fail(int status, std::string const& msg)
+{
+    g_last_error = msg;
+    return status;
+}
+
+int check_launch(char const* what)
+{
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        return fail(EB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    return EB_OK;
+}
+
+// Largest total cell count accepted with 32-bit indices: the reference's int_cast
+// rule plus headroom for the grid-stride arithmetic (idx + 4*stride, stride <= 2^20).
+constexpr int64_t kMax32 = std::numeric_limits<int32_t>::max() - (int64_t(1) << 22);
+
+int check_extents(int64_t n0, int64_t n1, int64_t n2, int index_bits)
+{
+    if (n0 < 0 || n1 < 0 || n2 < 0) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "negative extent");
+    }
+    if (index_bits != 32 && index_bits != 64) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "index_bits must be 32 or 64");
+    }
+    // Overflow-safe product check.
+    __int128 const total = static_cast<__int128>(n0) * n1 * n2;
+    __int128 const limit = index_bits == 32 ? kMax32 : std::numeric_limits<int64_t>::max() / 4;
+    if (total > limit) {
+        return fail(EB_ERR_INDEX_OVERFLOW, "extents do not fit the index type");
+    }
+    return EB_OK;
+}
+
+template <class T, class S>
+eb::PrimPtrs<T> prim_ptrs(S const* s)
+{
+    return {s->d, s->p, s->ux0, s->ux1, s->ux2};
+}
+
+template <class T, class S>
+eb::ConsPtrs<T> cons_ptrs(S const* s)
+{
+    return {s->d, s->e, s->mx0, s->mx1, s->mx2};
+}
+
+template <class S>
+bool prim_null(S const* s)
+{
+    return s == nullptr || s->d == nullptr || s->p == nullptr || s->ux0 == nullptr || s->ux1 == nullptr
+           || s->ux2 == nullptr;
+}
+
+template <class S>
+bool cons_null(S const* s)
+{
+    return s == nullptr || s->d == nullptr || s->e == nullptr || s->mx0 == nullptr || s->mx1 == nullptr
+           || s->mx2 == nullptr;
+}
+
+unsigned int pointwise_grid(int64_t n)
+{
+    int64_t const blocks = (n + 255) / 256;
+    return static_cast<unsigned int>(blocks < (int64_t(1) << 20) ? (blocks > 0 ? blocks : 1) : (int64_t(1) << 20));
+}
+
+unsigned int reduction_grid(int64_t n)
+{
+    // About eight resident blocks per CU on a 256-CU part; fewer for small inputs.
+    int64_t const blocks = (n + 1023) / 1024;
+    return static_cast<unsigned int>(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <class T, class I, class CS, class PS>
+int run_cons_to_prim(hipStream_t s, CS const* cons, PS const* prim, int64_t n0, int64_t n1, int64_t n2, T gamma)
+{
+    int64_t const n = n0 * n1 * n2;
+    if (n == 0) {
+        return EB_OK;
+    }
+    auto const c = cons_ptrs<T>(cons);
+    eb::ConsPtrs<T const> const cc {c.d, c.e, c.mx0, c.mx1, c.mx2};
+    hipLaunchKernelGGL((eb::k_cons_to_prim<T, I>), dim3(pointwise_grid(n)), dim3(256), 0, s, cc,
+                       prim_ptrs<T>(prim), static_cast<I>(n), static_cast<T>(gamma - 1));
+    return check_launch("cons_to_prim");
+}
+
+template <class T, class I, class PS, class CS>
+int run_prim_to_cons(hipStream_t s, PS const* prim, CS const* cons, int64_t n0, int64_t n1, int64_t n2, T gamma)
+{
+    int64_t const n = n0 * n1 * n2;
+    if (n == 0) {
+        return EB_OK;
+    }
+    auto const p = prim_ptrs<T>(prim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    hipLaunchKernelGGL((eb::k_prim_to_cons<T, I>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc,
+                       cons_ptrs<T>(cons), static_cast<I>(n), static_cast<T>(gamma - 1));
+    return check_launch("prim_to_cons");
+}
+
+template <class T>
+eb::DtSink<T> plain_sink(eb_context* ctx, T* dt_out)
+{
+    return eb::DtSink<T> {ctx->scratch, dt_out, nullptr, nullptr, nullptr, T(0)};
+}
+
+template <class T, class I, class PS>
+int run_time_step(
+        eb_context* ctx,
+        hipStream_t s,
+        PS const* prim,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        T gamma,
+        T dx0,
+        T dx1,
+        T dx2,
+        T* dt_dev)
+{
+    int64_t const n = n0 * n1 * n2;
+    auto const p = prim_ptrs<T>(prim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    // time_step.hpp:18-20: the inverse spacings are formed on the host.
+    T const invdx0 = T(1) / dx0;
+    T const invdx1 = T(1) / dx1;
+    T const invdx2 = T(1) / dx2;
+    hipLaunchKernelGGL((eb::k_time_step<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc, static_cast<I>(n),
+                       gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+    return check_launch("time_step");
+}
+
+struct Mesh
+{
+    template <class T>
+    static void faces(T dx0, T dx1, T dx2, T& ds0, T& ds1, T& ds2, T& dv)
+    {
+        // uniform_mesh.hpp:33-51
+        ds0 = dx1 * dx2;
+        ds1 = dx2 * dx0;
+        ds2 = dx0 * dx1;
+        dv = dx0 * dx1 * dx2;
+    }
+};
+
+template <class T, class I, int BY, bool Fused>
+void launch_tiled(
+        hipStream_t s,
+        eb::PrimPtrs<T const> const& pc,
+        eb::ConsPtrs<T> const& cu,
+        eb::PrimPtrs<T> const& pout,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        int kchunk,
+        T gamma,
+        T ds0,
+        T ds1,
+        T ds2,
+        eb::DtArg<T> const& dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        eb::DtSink<T> const& sink)
+{
+    int const tiles_x = static_cast<int>((n0 - 2 + 63) / 64);
+    int const tiles_y = static_cast<int>((n1 - 2 + BY - 1) / BY);
+    int const tiles_z = static_cast<int>((n2 - 2 + kchunk - 1) / kchunk);
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    hipLaunchKernelGGL((eb::k_godunov_tiled<T, I, BY, Fused>), dim3(nblk), dim3(64, BY), 0, s, pc, cu, pout,
+                       static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
+                       tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
+                       sink);
+}
+
+template <class T, class I, bool Fused>
+int dispatch_tiled(
+        hipStream_t s,
+        eb::PrimPtrs<T const> const& pc,
+        eb::ConsPtrs<T> const& cu,
+        eb::PrimPtrs<T> const& pout,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        T gamma,
+        T ds0,
+        T ds1,
+        T ds2,
+        eb::DtArg<T> const& dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        eb::DtSink<T> const& sink)
+{
+    int const by = g_tile_by.load();
+    int const kchunk = g_tile_kchunk.load();
+    // The block count must fit a 32-bit grid.
+    int64_t const tiles = ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by) * ((n2 - 2 + kchunk - 1) / kchunk);
+    if (tiles > std::numeric_limits<int32_t>::max()) {
+        return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
+    }
+    switch (by) {
+    case 4:
+        launch_tiled<T, I, 4, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0, invdx1,
+                                     invdx2, sink);
+        break;
+    case 8:
+        launch_tiled<T, I, 8, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0, invdx1,
+                                     invdx2, sink);
+        break;
+    case 16:
+        launch_tiled<T, I, 16, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0,
+                                      invdx1, invdx2, sink);
+        break;
+    default:
+        return fail(EB_ERR_UNSUPPORTED, "tile height must be 4, 8 or 16");
+    }
+    return check_launch(Fused ? "step_fused" : "godunov");
+}
+
+template <class T, class I, class PS, class CS>
+int run_godunov(
+        hipStream_t s,
+        PS const* prim,
+        CS const* cons,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        T gamma,
+        T dx0,
+        T dx1,
+        T dx2,
+        T dt,
+        T const* dt_dev,
+        T cfl,
+        bool reference_order)
+{
+    if (n0 < 3 || n1 < 3 || n2 < 3) {
+        return EB_OK; // no interior cell (godunov.hpp:40-43 gives an empty range)
+    }
+    T ds0;
+    T ds1;
+    T ds2;
+    T dv;
+    Mesh::faces(dx0, dx1, dx2, ds0, ds1, ds2, dv);
+    eb::DtArg<T> const dta {dt / dv, dt_dev, cfl, dv};
+    auto const p = prim_ptrs<T>(prim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    auto const cu = cons_ptrs<T>(cons);
+    if (reference_order) {
+        int64_t const blocks = ((n0 - 2 + 63) / 64) * ((n1 - 2 + 3) / 4) * (n2 - 2);
+        if (blocks > std::numeric_limits<int32_t>::max()) {
+            return fail(EB_ERR_INDEX_OVERFLOW, "grid too large for the reference-order kernel");
+        }
+        hipLaunchKernelGGL((eb::k_godunov_naive<T, I>), dim3(static_cast<unsigned int>(blocks)), dim3(64, 4), 0, s,
+                           pc, cu, static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma,
+                           static_cast<T>(gamma - 1), ds0, ds1, ds2, dta);
+        return check_launch("godunov_reference_order");
+    }
+    eb::DtSink<T> const none {nullptr, nullptr, nullptr, nullptr, nullptr, T(0)};
+    return dispatch_tiled<T, I, false>(s, pc, cu, eb::PrimPtrs<T> {}, n0, n1, n2, gamma, ds0, ds1, ds2, dta, T(0),
+                                       T(0), T(0), none);
+}
+
+template <class T, class I, class CS, class PS>
+int run_ghost_refresh(
+        eb_context* ctx,
+        hipStream_t s,
+        CS const* cons,
+        PS const* prim_out,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        T gamma,
+        T dx0,
+        T dx1,
+        T dx2,
+        T* ghost_min)
+{
+    int64_t const n = n0 * n1 * n2;
+    int64_t const inner = (n0 > 2 ? n0 - 2 : 0) * (n1 > 2 ? n1 - 2 : 0) * (n2 > 2 ? n2 - 2 : 0);
+    auto const c = cons_ptrs<T>(cons);
+    eb::ConsPtrs<T const> const cc {c.d, c.e, c.mx0, c.mx1, c.mx2};
+    hipLaunchKernelGGL((eb::k_ghost_refresh<T, I>), dim3(reduction_grid(n - inner)), dim3(256), 0, s, cc,
+                       prim_ptrs<T>(prim_out), static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma,
+                       static_cast<T>(gamma - 1), T(1) / dx0, T(1) / dx1, T(1) / dx2,
+                       plain_sink<T>(ctx, ghost_min));
+    return check_launch("ghost_refresh");
+}
+
+template <class T, class I, class PS, class CS>
+int run_step_fused(
+        eb_context* ctx,
+        hipStream_t s,
+        PS const* prim_in,
+        CS const* cons,
+        PS const* prim_out,
+        int64_t n0,
+        int64_t n1,
+        int64_t n2,
+        T gamma,
+        T dx0,
+        T dx1,
+        T dx2,
+        T cfl,
+        T const* dt_in,
+        T const* ghost_min,
+        T* dt_out,
+        double* time_acc)
+{
+    T ds0;
+    T ds1;
+    T ds2;
+    T dv;
+    Mesh::faces(dx0, dx1, dx2, ds0, ds1, ds2, dv);
+    eb::DtArg<T> const dta {T(0), dt_in, cfl, dv};
+    auto const p = prim_ptrs<T>(prim_in);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    eb::DtSink<T> const sink {ctx->scratch, dt_out, ghost_min, time_acc, dt_in, cfl};
+    return dispatch_tiled<T, I, true>(s, pc, cons_ptrs<T>(cons), prim_ptrs<T>(prim_out), n0, n1, n2, gamma, ds0,
+                                      ds1, ds2, dta, T(1) / dx0, T(1) / dx1, T(1) / dx2, sink);
+}
+
+} // namespace
+
+// ======================================================================= ABI
+
+extern "C" {
+
+int eb_context_create(eb_context** ctx)
+{
+    if (ctx == nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "ctx is null");
+    }
+    *ctx = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        return fail(EB_ERR_HIP, "hipGetDevice failed");
+    }
+    eb::MinScratch* scratch = nullptr;
+    double* dt_slot = nullptr;
+    if (hipMalloc(&scratch, sizeof(eb::MinScratch)) != hipSuccess) {
+        return fail(EB_ERR_OUT_OF_MEMORY, "hipMalloc of reduction scratch failed");
+    }
+    if (hipMalloc(&dt_slot, sizeof(double)) != hipSuccess) {
+        (void)hipFree(scratch);
+        return fail(EB_ERR_OUT_OF_MEMORY, "hipMalloc of dt slot failed");
+    }
+    hipLaunchKernelGGL(eb::min_scratch_init, dim3(1), dim3(64), 0, nullptr, scratch, 1);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(scratch);
+        (void)hipFree(dt_slot);
+        return fail(EB_ERR_HIP, "reduction scratch initialisation failed");
+    }
+    *ctx = new eb_context {dev, scratch, dt_slot};
+    return EB_OK;
+}
+
+int eb_context_destroy(eb_context* ctx)
+{
+    if (ctx == nullptr) {
+        return EB_OK;
+    }
+    bool const ok = hipFree(ctx->scratch) == hipSuccess && hipFree(ctx->dt_slot) == hipSuccess;
+    int const rc = ok ? EB_OK : fail(EB_ERR_HIP, "hipFree failed");
+    delete ctx;
+    return rc;
+}
+
+const char* eb_status_string(int status)
+{
+    switch (status) {
+    case EB_OK: return "ok";
+    case EB_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case EB_ERR_INDEX_OVERFLOW: return "index overflow";
+    case EB_ERR_HIP: return "HIP error";
+    case EB_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case EB_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+const char* eb_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+int eb_set_godunov_tile(int by, int kchunk)
+{
+    if (by != 4 && by != 8 && by != 16) {
+        return fail(EB_ERR_UNSUPPORTED, "tile height must be 4, 8 or 16");
+    }
+    if (kchunk < 1) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 1");
+    }
+    g_tile_by.store(by);
+    g_tile_kchunk.store(kchunk);
+    return EB_OK;
+}
+
+int eb_get_godunov_tile(int* by, int* kchunk)
+{
+    if (by == nullptr || kchunk == nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null output");
+    }
+    *by = g_tile_by.load();
+    *kchunk = g_tile_kchunk.load();
+    return EB_OK;
+}
+
+#define EB_CHECK(expr)            \
+    do {                          \
+        int const rc_ = (expr);   \
+        if (rc_ != EB_OK) {       \
+            return rc_;           \
+        }                         \
+    } while (0)
+
+#define EB_DISPATCH_INDEX(bits, CALL32, CALL64) ((bits) == 32 ? (CALL32) : (CALL64))
+
+#define EB_DEFINE_ABI(SUF, T)                                                                                     \
+    int eb_cons_to_prim_##SUF(hipStream_t stream, const eb_cons_##SUF* cons, const eb_prim_##SUF* prim,          \
+                              int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma)                        \
+    {                                                                                                             \
+        if (cons_null(cons) || prim_null(prim)) {                                                                 \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits, (run_cons_to_prim<T, int32_t>(stream, cons, prim, n0, n1, n2, gamma)), \
+                                 (run_cons_to_prim<T, int64_t>(stream, cons, prim, n0, n1, n2, gamma)));          \
+    }                                                                                                             \
+    int eb_prim_to_cons_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons,          \
+                              int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma)                        \
+    {                                                                                                             \
+        if (cons_null(cons) || prim_null(prim)) {                                                                 \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits, (run_prim_to_cons<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma)), \
+                                 (run_prim_to_cons<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma)));          \
+    }                                                                                                             \
+    int eb_time_step_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0,           \
+                           int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T* dt_dev)      \
+    {                                                                                                             \
+        if (ctx == nullptr || prim_null(prim) || dt_dev == nullptr) {                                             \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(                                                                                 \
+                index_bits,                                                                                       \
+                (run_time_step<T, int32_t>(ctx, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)),         \
+                (run_time_step<T, int64_t>(ctx, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)));        \
+    }                                                                                                             \
+    int eb_time_step_sync_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0,      \
+                                int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2,             \
+                                T* dt_host)                                                                       \
+    {                                                                                                             \
+        if (dt_host == nullptr) {                                                                                 \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null dt_host");                                                 \
+        }                                                                                                         \
+        if (ctx == nullptr) {                                                                                     \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null ctx");                                                     \
+        }                                                                                                         \
+        T* const dt_dev = reinterpret_cast<T*>(ctx->dt_slot);                                                     \
+        EB_CHECK(eb_time_step_##SUF(ctx, stream, prim, n0, n1, n2, index_bits, gamma, dx0, dx1, dx2, dt_dev));   \
+        if (hipMemcpyAsync(dt_host, dt_dev, sizeof(T), hipMemcpyDeviceToHost, stream) != hipSuccess               \
+            || hipStreamSynchronize(stream) != hipSuccess) {                                                      \
+            return fail(EB_ERR_HIP, "dt copy to host failed");                                                    \
+        }                                                                                                         \
+        return EB_OK;                                                                                             \
+    }                                                                                                             \
+    int eb_godunov_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons, int64_t n0,   \
+                         int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T dt)              \
+    {                                                                                                             \
+        if (cons_null(cons) || prim_null(prim)) {                                                                 \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits,                                                                      \
+                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          dt, nullptr, T(0), false)),                             \
+                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          dt, nullptr, T(0), false)));                            \
+    }                                                                                                             \
+    int eb_godunov_dev_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons,           \
+                             int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2,    \
+                             const T* dt_dev, T cfl)                                                              \
+    {                                                                                                             \
+        if (cons_null(cons) || prim_null(prim) || dt_dev == nullptr) {                                            \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits,                                                                      \
+                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          T(0), dt_dev, cfl, false)),                             \
+                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          T(0), dt_dev, cfl, false)));                            \
+    }                                                                                                             \
+    int eb_godunov_reference_order_##SUF(hipStream_t stream, const eb_prim_##SUF* prim,                          \
+                                         const eb_cons_##SUF* cons, int64_t n0, int64_t n1, int64_t n2,           \
+                                         int index_bits, T gamma, T dx0, T dx1, T dx2, T dt)                      \
+    {                                                                                                             \
+        if (cons_null(cons) || prim_null(prim)) {                                                                 \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits,                                                                      \
+                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          dt, nullptr, T(0), true)),                              \
+                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
+                                                          dt, nullptr, T(0), true)));                             \
+    }                                                                                                             \
+    int eb_ghost_refresh_##SUF(eb_context* ctx, hipStream_t stream, const eb_cons_##SUF* cons,                   \
+                               const eb_prim_##SUF* prim_out, int64_t n0, int64_t n1, int64_t n2,                 \
+                               int index_bits, T gamma, T dx0, T dx1, T dx2, T* ghost_min_dev)                    \
+    {                                                                                                             \
+        if (ctx == nullptr || cons_null(cons) || prim_null(prim_out) || ghost_min_dev == nullptr) {               \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        return EB_DISPATCH_INDEX(index_bits,                                                                      \
+                                 (run_ghost_refresh<T, int32_t>(ctx, stream, cons, prim_out, n0, n1, n2, gamma,   \
+                                                                dx0, dx1, dx2, ghost_min_dev)),                   \
+                                 (run_ghost_refresh<T, int64_t>(ctx, stream, cons, prim_out, n0, n1, n2, gamma,   \
+                                                                dx0, dx1, dx2, ghost_min_dev)));                  \
+    }                                                                                                             \
+    int eb_step_fused_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim_in,                   \
+                            const eb_cons_##SUF* cons, const eb_prim_##SUF* prim_out, int64_t n0, int64_t n1,    \
+                            int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T cfl, const T* dt_in,     \
+                            const T* ghost_min, T* dt_out, double* time_acc)                                      \
+    {                                                                                                             \
+        if (ctx == nullptr || prim_null(prim_in) || cons_null(cons) || prim_null(prim_out) || dt_in == nullptr    \
+            || dt_out == nullptr) {                                                                               \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
+        }                                                                                                         \
+        if (prim_in->d == prim_out->d || dt_in == dt_out) {                                                       \
+            return fail(EB_ERR_INVALID_ARGUMENT, "prim_out / dt_out must not alias the inputs");                  \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        if (n0 < 3 || n1 < 3 || n2 < 3) {                                                                         \
+            return fail(EB_ERR_INVALID_ARGUMENT, "fused step needs at least one interior cell");                  \
+        }                                                                                                         \
+        return EB_DISPATCH_INDEX(index_bits,                                                                      \
+                                 (run_step_fused<T, int32_t>(ctx, stream, prim_in, cons, prim_out, n0, n1, n2,    \
+                                                             gamma, dx0, dx1, dx2, cfl, dt_in, ghost_min, dt_out, \
+                                                             time_acc)),                                          \
+                                 (run_step_fused<T, int64_t>(ctx, stream, prim_in, cons, prim_out, n0, n1, n2,    \
+                                                             gamma, dx0, dx1, dx2, cfl, dt_in, ghost_min, dt_out, \
+                                                             time_acc)));                                         \
+    }                                                                                                             \
+    int eb_init_implode_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0, int64_t n1, int64_t n2, \
+                              int index_bits, T dx0, T dx1, T dx2)                                                \
+    {                                                                                                             \
+        if (prim_null(prim)) {                                                                                    \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        int64_t const n = n0 * n1 * n2;                                                                           \
+        if (n == 0) {                                                                                             \
+            return EB_OK;                                                                                         \
+        }                                                                                                         \
+        auto const p = prim_ptrs<T>(prim);                                                                        \
+        if (index_bits == 32) {                                                                                   \
+            hipLaunchKernelGGL((eb::k_init_implode<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,   \
+                               p, int32_t(n0), int32_t(n1), int32_t(n2), dx0, dx1, dx2);                          \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((eb::k_init_implode<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,   \
+                               p, n0, n1, n2, dx0, dx1, dx2);                                                     \
+        }                                                                                                         \
+        return check_launch("init_implode");                                                                      \
+    }                                                                                                             \
+    int eb_init_diagonal_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0, int64_t n1,            \
+                               int64_t n2, int index_bits, T dx0, T dx1, T dx2, T offset, T d_hi, T p_hi,         \
+                               T d_lo, T p_lo)                                                                    \
+    {                                                                                                             \
+        if (prim_null(prim)) {                                                                                    \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
+        int64_t const n = n0 * n1 * n2;                                                                           \
+        if (n == 0) {                                                                                             \
+            return EB_OK;                                                                                         \
+        }                                                                                                         \
+        auto const p = prim_ptrs<T>(prim);                                                                        \
+        if (index_bits == 32) {                                                                                   \
+            hipLaunchKernelGGL((eb::k_init_diagonal<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,  \
+                               p, int32_t(n0), int32_t(n1), int32_t(n2), dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, \
+                               p_lo);                                                                             \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((eb::k_init_diagonal<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,  \
+                               p, n0, n1, n2, dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo);                     \
+        }                                                                                                         \
+        return check_launch("init_diagonal");                                                                     \
+    }                                                                                                             \
+    int eb_init_random_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n, int index_bits,           \
+                             uint64_t seed, double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma)   \
+    {                                                                                                             \
+        if (prim_null(prim)) {                                                                                    \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
+        }                                                                                                         \
+        EB_CHECK(check_extents(n, 1, 1, index_bits));                                                             \
+        if (n == 0) {                                                                                             \
+            return EB_OK;                                                                                         \
+        }                                                                                                         \
+        auto const p = prim_ptrs<T>(prim);                                                                        \
+        if (index_bits == 32) {                                                                                   \
+            hipLaunchKernelGGL((eb::k_init_random<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream, p, \
+                               int32_t(n), static_cast<unsigned long long>(seed), d_lo, d_hi, p_lo, p_hi,         \
+                               u_sigma);                                                                          \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((eb::k_init_random<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream, p, \
+                               n, static_cast<unsigned long long>(seed), d_lo, d_hi, p_lo, p_hi, u_sigma);        \
+        }                                                                                                         \
+        return check_launch("init_random");                                                                       \
+    }
+
+EB_DEFINE_ABI(f64, double)
+EB_DEFINE_ABI(f32, float)
+
+#undef EB_DEFINE_ABI
+#undef EB_DISPATCH_INDEX
+#undef EB_CHECK
+
+} // extern "C"

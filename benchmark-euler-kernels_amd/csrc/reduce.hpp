@@ -1,0 +1,162 @@
+// Min-reduction machinery for the CFL time step: wavefront shuffles, an LDS
+// combine across the waves of a block, and a single-pass grid completion in which
+// the last block to arrive publishes the result to device memory and re-arms the
+// scratch words for the next launch. No host synchronisation is involved and no
+// memset is needed between launches, so the sequence is safe inside a hipGraph.
+//
+// Semantics follow the reference's Kokkos::Min<T> reduction (time_step.hpp:37-39):
+// identity = largest finite value, candidate kept only when it compares strictly
+// smaller, so NaN candidates never enter. The minimum of non-NaN values is exact,
+// hence independent of the combine order.
+#pragma once
+
+#include <cfloat>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace eb {
+
+// Scratch words owned by an eb_context (one per concurrent reduction stream).
+struct MinScratch
+{
+    unsigned long long acc; // ordered encoding of the running minimum
+    unsigned int count;     // blocks that have contributed
+    unsigned int pad;
+};
+
+template <class T>
+struct Limits;
+
+template <>
+struct Limits<double>
+{
+    static constexpr double max() { return DBL_MAX; }
+};
+
+template <>
+struct Limits<float>
+{
+    static constexpr float max() { return FLT_MAX; }
+};
+
+// Order-preserving map from IEEE bits to unsigned integers (non-NaN inputs).
+__device__ __forceinline__ unsigned long long ord_encode(double x)
+{
+    unsigned long long const b = static_cast<unsigned long long>(__double_as_longlong(x));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ unsigned long long ord_encode(float x)
+{
+    unsigned int const b = __float_as_uint(x);
+    return static_cast<unsigned long long>((b >> 31) ? ~b : (b | 0x80000000u));
+}
+
+template <class T>
+__device__ __forceinline__ T ord_decode(unsigned long long e);
+
+template <>
+__device__ __forceinline__ double ord_decode<double>(unsigned long long e)
+{
+    unsigned long long const b = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+template <>
+__device__ __forceinline__ float ord_decode<float>(unsigned long long e)
+{
+    unsigned int const u = static_cast<unsigned int>(e);
+    unsigned int const b = (u >> 31) ? (u & 0x7fffffffu) : ~u;
+    return __uint_as_float(b);
+}
+
+// Kokkos::Min join: keep the incumbent unless the candidate is strictly smaller.
+template <class T>
+__device__ __forceinline__ T min_join(T incumbent, T candidate)
+{
+    return (candidate < incumbent) ? candidate : incumbent;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v = min_join(v, __shfl_xor(v, off, 64));
+    }
+    return v;
+}
+
+// Block-wide minimum; every thread must call it. Result valid in thread 0.
+template <class T, int MaxWaves>
+__device__ __forceinline__ T block_min(T v)
+{
+    __shared__ T partial[MaxWaves];
+    int const lid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    int const nthreads = blockDim.x * blockDim.y * blockDim.z;
+    int const wave = lid >> 6;
+    int const lane = lid & 63;
+    v = wave_min(v);
+    if (lane == 0) {
+        partial[wave] = v;
+    }
+    __syncthreads();
+    T r = Limits<T>::max();
+    if (lid == 0) {
+        int const nw = (nthreads + 63) >> 6;
+        for (int w = 0; w < nw; ++w) {
+            r = min_join(r, partial[w]);
+        }
+    }
+    return r;
+}
+
+// Arguments for the grid completion performed by the last block.
+template <class T>
+struct DtSink
+{
+    MinScratch* scratch;     // self-resetting accumulator words
+    T* dt_out;               // receives the raw minimum (no CFL factor)
+    T const* extra_min;      // optional constant candidate merged at the end (nullable)
+    double* time_acc;        // optional FP64 simulated-time accumulator (nullable)
+    T const* dt_in;          // dt of the step just taken (for time_acc), raw
+    T cfl;                   // time_acc += double(cfl * dt_in[0])
+};
+
+// Called by thread 0 of every block with that block's minimum.
+template <class T>
+__device__ __forceinline__ void grid_min_commit(DtSink<T> const& sink, T block_value, unsigned int nblocks)
+{
+    atomicMin(&sink.scratch->acc, ord_encode(block_value));
+    __threadfence();
+    unsigned int const arrived = atomicAdd(&sink.scratch->count, 1u);
+    if (arrived == nblocks - 1) {
+        __threadfence();
+        // All-ones sits above every encoding of either precision, so the words can
+        // be shared by FP32 and FP64 launches.
+        unsigned long long const enc = atomicExch(&sink.scratch->acc, 0xffffffffffffffffull);
+        T m = ord_decode<T>(enc);
+        if (sink.extra_min != nullptr) {
+            m = min_join(m, *sink.extra_min);
+        }
+        sink.dt_out[0] = m;
+        if (sink.time_acc != nullptr && sink.dt_in != nullptr) {
+            T const dt_step = sink.cfl * sink.dt_in[0];
+            sink.time_acc[0] += static_cast<double>(dt_step);
+        }
+        atomicExch(&sink.scratch->count, 0u);
+    }
+}
+
+__global__ void min_scratch_init(MinScratch* s, int n)
+{
+    int const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        s[t].acc = 0xffffffffffffffffull; // replaced by the first commit's atomicMin
+        s[t].count = 0u;
+        s[t].pad = 0u;
+    }
+}
+
+} // namespace eb

@@ -1,0 +1,165 @@
+/* C ABI of the MI355X (gfx950) Godunov step for the 3-D compressible Euler
+ * equations: HLLC Riemann solver, perfect-gas EOS, uniform mesh, first order.
+ *
+ * Every entry point takes plain device pointers and sizes, enqueues work on the
+ * given HIP stream, never throws, and returns an eb_status code. Arrays follow the
+ * reference's SoA layout (euler_operators/euler_arrays.hpp:77-85, 228-236): five
+ * separate fields per state, layout_left, linear index i + n0*(j + n1*k), extents
+ * n0 x n1 x n2 INCLUDING the one-cell ghost layer. The caller owns every array.
+ *
+ * index_bits selects the kernels' index type: 32 (the reference's `int` /
+ * `unsigned` index_t, benchmarks/index_type.hpp:3-4) or 64 (int64_t). With 32 the
+ * total cell count must fit a signed 32-bit index (the reference's int_cast rule,
+ * benchmarks/benchmark_utils.hpp:10-17), else EB_ERR_INDEX_OVERFLOW.
+ *
+ * Each function exists for f64 (real_t = double) and f32 (real_t = float), the
+ * reference's real_type switch (benchmarks/real_type.hpp:3-4).
+ */
+#ifndef EULER_CAPI_H
+#define EULER_CAPI_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum eb_status {
+    EB_OK = 0,
+    EB_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative extent, bad option */
+    EB_ERR_INDEX_OVERFLOW = 2,   /* extents do not fit the requested index type */
+    EB_ERR_HIP = 3,              /* a HIP call or kernel launch failed */
+    EB_ERR_OUT_OF_MEMORY = 4,    /* device allocation of context scratch failed */
+    EB_ERR_UNSUPPORTED = 5       /* option outside the supported set */
+} eb_status;
+
+/* EulerPrim / EulerCons field order (euler_arrays.hpp:8-26). */
+typedef struct eb_prim_f64 { double* d; double* p; double* ux0; double* ux1; double* ux2; } eb_prim_f64;
+typedef struct eb_cons_f64 { double* d; double* e; double* mx0; double* mx1; double* mx2; } eb_cons_f64;
+typedef struct eb_prim_f32 { float* d; float* p; float* ux0; float* ux1; float* ux2; } eb_prim_f32;
+typedef struct eb_cons_f32 { float* d; float* e; float* mx0; float* mx1; float* mx2; } eb_cons_f32;
+
+/* Device-side scratch for the single-pass CFL reduction (replaces the Kokkos
+ * reduction buffer behind time_step.hpp:22-39). One context serves one device;
+ * reductions enqueued through one context must not overlap in time (use one
+ * context per concurrently running stream). */
+typedef struct eb_context eb_context;
+
+int eb_context_create(eb_context** ctx);
+int eb_context_destroy(eb_context* ctx);
+
+const char* eb_status_string(int status);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* eb_last_error(void);
+
+/* Godunov kernel launch shape: tile height `by` in {4, 8, 16} and planes per block
+ * along k (`kchunk` >= 1). Process-wide; defaults 8 and 64. */
+int eb_set_godunov_tile(int by, int kchunk);
+int eb_get_godunov_tile(int* by, int* kchunk);
+
+/* ---- cons_to_prim.hpp:7-33 --------------------------------------------- */
+int eb_cons_to_prim_f64(hipStream_t stream, const eb_cons_f64* cons, const eb_prim_f64* prim,
+                        int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma);
+int eb_cons_to_prim_f32(hipStream_t stream, const eb_cons_f32* cons, const eb_prim_f32* prim,
+                        int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma);
+
+/* ---- prim_to_cons.hpp:7-33 --------------------------------------------- */
+int eb_prim_to_cons_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* cons,
+                        int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma);
+int eb_prim_to_cons_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* cons,
+                        int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma);
+
+/* ---- time_step.hpp:8-41 -------------------------------------------------
+ * Asynchronous form: the raw dt (min over all cells, ghosts included, of
+ * 1/sum_d((c+|u_d|)/dx_d); no CFL factor) is written to dt_dev[0] on the device.
+ * Blocking form: same value returned to *dt_host, as the reference does. */
+int eb_time_step_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64* prim,
+                     int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                     double dx0, double dx1, double dx2, double* dt_dev);
+int eb_time_step_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim,
+                     int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                     float dx0, float dx1, float dx2, float* dt_dev);
+int eb_time_step_sync_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64* prim,
+                          int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                          double dx0, double dx1, double dx2, double* dt_host);
+int eb_time_step_sync_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim,
+                          int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                          float dx0, float dx1, float dx2, float* dt_host);
+
+/* ---- godunov.hpp:15-92 (hllc, PerfectGas, UniformMesh3d) ----------------
+ * Updates cons in place on interior cells from the (const) primitives with a
+ * host time step dt, like the reference. prim and cons must not alias. Extents
+ * below 3 in any direction leave cons unchanged. Tiled, face-once kernel. */
+int eb_godunov_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* cons,
+                   int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                   double dx0, double dx1, double dx2, double dt);
+int eb_godunov_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* cons,
+                   int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                   float dx0, float dx1, float dx2, float dt);
+/* Same update with dt = cfl * dt_dev[0] read on the device (no host sync). */
+int eb_godunov_dev_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* cons,
+                       int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                       double dx0, double dx1, double dx2, const double* dt_dev, double cfl);
+int eb_godunov_dev_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* cons,
+                       int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                       float dx0, float dx1, float dx2, const float* dt_dev, float cfl);
+/* Reference-order kernel (six Riemann solves per cell), for validation. */
+int eb_godunov_reference_order_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* cons,
+                                   int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                                   double dx0, double dx1, double dx2, double dt);
+int eb_godunov_reference_order_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* cons,
+                                   int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                                   float dx0, float dx1, float dx2, float dt);
+
+/* ---- fused step (extension of euler_simulation.cpp:56-68) ---------------
+ * One launch performing godunov(prim_in, cons, cfl*dt_in[0]) followed by
+ * cons_to_prim(cons -> prim_out) on interior cells and the next time_step over
+ * prim_out, whose raw dt is written to dt_out[0]. Ghost cells of prim_out and
+ * their dt contribution come from eb_ghost_refresh (ghost_min = its output).
+ * time_acc (nullable) accumulates cfl*dt_in[0] in FP64. prim_out must differ
+ * from prim_in; dt_out must differ from dt_in. */
+int eb_ghost_refresh_f64(eb_context* ctx, hipStream_t stream, const eb_cons_f64* cons, const eb_prim_f64* prim_out,
+                         int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                         double dx0, double dx1, double dx2, double* ghost_min_dev);
+int eb_ghost_refresh_f32(eb_context* ctx, hipStream_t stream, const eb_cons_f32* cons, const eb_prim_f32* prim_out,
+                         int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                         float dx0, float dx1, float dx2, float* ghost_min_dev);
+int eb_step_fused_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64* prim_in, const eb_cons_f64* cons,
+                      const eb_prim_f64* prim_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                      double gamma, double dx0, double dx1, double dx2, double cfl,
+                      const double* dt_in, const double* ghost_min, double* dt_out, double* time_acc);
+int eb_step_fused_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim_in, const eb_cons_f32* cons,
+                      const eb_prim_f32* prim_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                      float gamma, float dx0, float dx1, float dx2, float cfl,
+                      const float* dt_in, const float* ghost_min, float* dt_out, double* time_acc);
+
+/* ---- init_implode.hpp:7-45 (sphere r < 0.25 about the centre) ---------- */
+int eb_init_implode_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n0, int64_t n1, int64_t n2,
+                        int index_bits, double dx0, double dx1, double dx2);
+int eb_init_implode_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n0, int64_t n1, int64_t n2,
+                        int index_bits, float dx0, float dx1, float dx2);
+/* Diagonal interface x0+x1+x2 > offset -> (d_hi, p_hi) else (d_lo, p_lo), u = 0,
+ * cell centres ((i-1)+0.5)*dx on the unit cube. No reference counterpart: the
+ * measured implosion configuration asks for it. */
+int eb_init_diagonal_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n0, int64_t n1, int64_t n2,
+                         int index_bits, double dx0, double dx1, double dx2, double offset,
+                         double d_hi, double p_hi, double d_lo, double p_lo);
+int eb_init_diagonal_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n0, int64_t n1, int64_t n2,
+                         int index_bits, float dx0, float dx1, float dx2, float offset,
+                         float d_hi, float p_hi, float d_lo, float p_lo);
+/* Seeded synthetic state (counter-based, launch-shape independent):
+ * rho ~ U[d_lo,d_hi), p ~ U[p_lo,p_hi), u_k ~ N(0,u_sigma). No reference
+ * counterpart (the reference benchmarks use uniform states,
+ * benchmarks/benchmark_utils.hpp:53-87). */
+int eb_init_random_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n, int index_bits, uint64_t seed,
+                       double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma);
+int eb_init_random_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n, int index_bits, uint64_t seed,
+                       double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* EULER_CAPI_H */
