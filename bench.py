@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline godunov only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--tile", type=str, default=None, help="godunov tile 'by,kchunk'")
+    ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
     return ap.parse_args()
 
 
@@ -112,6 +112,12 @@ def cpu_baseline(n: int, budget_s: float = 10.0):
     }
 
 
+def kernel_name(cfg) -> str:
+    variant, by, _ = cfg
+    kind = "k_godunov_faces" if variant == 0 else "k_godunov_tiled"
+    return f"eb::{kind}<double, long, {by}, false>"
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -124,9 +130,8 @@ def main():
     from __graft_entry__ import load_package
 
     eb = load_package()
-    if args.tile:
-        by, kc = (int(v) for v in args.tile.split(","))
-        eb.set_godunov_tile(by, kc)
+    if args.config:
+        eb.set_godunov_config(*(int(v) for v in args.config.split(",")))
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     ex = eb.ExecSpace(dev)
@@ -172,7 +177,7 @@ def main():
     extra = {
         "godunov_f64_ms_event": round(ms, 4),
         "godunov_f64_Mcells_per_s_ref_convention": round(n_arr / (ms * 1e-3) / 1e6, 1),
-        "tile": list(eb.get_godunov_tile()),
+        "godunov_config": list(eb.get_godunov_config()),
     }
 
     if not args.quick:
@@ -266,7 +271,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "eb::k_godunov_tiled<double, long, %d, false>" % eb.get_godunov_tile()[0],
+            "kernel": kernel_name(eb.get_godunov_config()),
             "bytes_per_launch": bytes_godunov,
             "avg_launch_ms": round(ms, 4),
         },

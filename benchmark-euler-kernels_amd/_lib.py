@@ -57,8 +57,8 @@ def _signatures():
     sig = {
         "eb_context_create": [_P(_vp)],
         "eb_context_destroy": [_vp],
-        "eb_set_godunov_tile": [_int, _int],
-        "eb_get_godunov_tile": [_P(_int), _P(_int)],
+        "eb_set_godunov_config": [_int, _int, _int],
+        "eb_get_godunov_config": [_P(_int), _P(_int), _P(_int)],
     }
     for suf, real in (("f64", _f64), ("f32", _f32)):
         ext = [_i64, _i64, _i64, _int]

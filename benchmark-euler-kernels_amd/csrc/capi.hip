@@ -22,8 +22,18 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Godunov launch configuration (process-wide): kernel variant, tile height, planes
+// per block. Variant 0 = face-parallel waves, 1 = cell-parallel waves (default,
+// fastest measured at 512^3 on MI355X), 2 = streamed with a producer wave.
+std::atomic<int> g_variant {1};
 std::atomic<int> g_tile_by {8};
-std::atomic<int> g_tile_kchunk {64};
+std::atomic<int> g_tile_kchunk {128};
+
+bool config_supported(int variant, int by)
+{
+    return (variant == 0 && (by == 2 || by == 4)) || (variant == 1 && (by == 4 || by == 8 || by == 16))
+           || (variant == 2 && (by == 4 || by == 8));
+}
 
 int fail(int status, std::string const& msg)
 {
@@ -175,7 +185,14 @@ struct Mesh
     }
 };
 
-template <class T, class I, int BY, bool Fused>
+template <class T>
+bool aligned16(eb::PrimPtrs<T const> const& p)
+{
+    auto a = [](void const* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+    return a(p.d) && a(p.p) && a(p.ux0) && a(p.ux1) && a(p.ux2);
+}
+
+template <int Variant, class T, class I, int BY, bool Fused>
 void launch_tiled(
         hipStream_t s,
         eb::PrimPtrs<T const> const& pc,
@@ -199,10 +216,22 @@ void launch_tiled(
     int const tiles_y = static_cast<int>((n1 - 2 + BY - 1) / BY);
     int const tiles_z = static_cast<int>((n2 - 2 + kchunk - 1) / kchunk);
     int const nblk = tiles_x * tiles_y * tiles_z;
-    hipLaunchKernelGGL((eb::k_godunov_tiled<T, I, BY, Fused>), dim3(nblk), dim3(64, BY), 0, s, pc, cu, pout,
-                       static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
-                       tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
-                       sink);
+    if constexpr (Variant == 2) {
+        hipLaunchKernelGGL((eb::k_godunov_stream<T, I, BY, Fused>), dim3(nblk), dim3(64, BY + 1), 0, s, pc, cu, pout,
+                           static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
+                           tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
+                           sink);
+    } else if constexpr (Variant == 0) {
+        hipLaunchKernelGGL((eb::k_godunov_faces<T, I, BY, Fused>), dim3(nblk), dim3(64, 3 * BY), 0, s, pc, cu, pout,
+                           static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
+                           tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
+                           sink);
+    } else {
+        hipLaunchKernelGGL((eb::k_godunov_tiled<T, I, BY, Fused>), dim3(nblk), dim3(64, BY), 0, s, pc, cu, pout,
+                           static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
+                           tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
+                           sink);
+    }
 }
 
 template <class T, class I, bool Fused>
@@ -224,29 +253,41 @@ int dispatch_tiled(
         T invdx2,
         eb::DtSink<T> const& sink)
 {
+    int const variant = g_variant.load();
     int const by = g_tile_by.load();
     int const kchunk = g_tile_kchunk.load();
-    // The block count must fit a 32-bit grid.
+    // The block count must fit a 32-bit grid and a plane must fit 32-bit offsets.
     int64_t const tiles = ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by) * ((n2 - 2 + kchunk - 1) / kchunk);
     if (tiles > std::numeric_limits<int32_t>::max()) {
         return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
     }
-    switch (by) {
-    case 4:
-        launch_tiled<T, I, 4, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0, invdx1,
-                                     invdx2, sink);
-        break;
-    case 8:
-        launch_tiled<T, I, 8, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0, invdx1,
-                                     invdx2, sink);
-        break;
-    case 16:
-        launch_tiled<T, I, 16, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0,
-                                      invdx1, invdx2, sink);
-        break;
-    default:
-        return fail(EB_ERR_UNSUPPORTED, "tile height must be 4, 8 or 16");
+    if (static_cast<__int128>(n0) * n1 * static_cast<int64_t>(sizeof(T)) >= (static_cast<__int128>(1) << 32)) {
+        return fail(EB_ERR_UNSUPPORTED, "a k-plane must hold fewer than 2^32 bytes per field");
     }
+#define EB_LAUNCH(V, B)                                                                                          \
+    launch_tiled<V, T, I, B, Fused>(s, pc, cu, pout, n0, n1, n2, kchunk, gamma, ds0, ds1, ds2, dta, invdx0, invdx1, \
+                                    invdx2, sink)
+    if (variant == 2 && !aligned16(pc)) {
+        return fail(EB_ERR_UNSUPPORTED, "the streamed variant needs 16-byte aligned primitive fields");
+    }
+    if (variant == 2 && by == 4) {
+        EB_LAUNCH(2, 4);
+    } else if (variant == 2 && by == 8) {
+        EB_LAUNCH(2, 8);
+    } else if (variant == 0 && by == 2) {
+        EB_LAUNCH(0, 2);
+    } else if (variant == 0 && by == 4) {
+        EB_LAUNCH(0, 4);
+    } else if (variant == 1 && by == 4) {
+        EB_LAUNCH(1, 4);
+    } else if (variant == 1 && by == 8) {
+        EB_LAUNCH(1, 8);
+    } else if (variant == 1 && by == 16) {
+        EB_LAUNCH(1, 16);
+    } else {
+        return fail(EB_ERR_UNSUPPORTED, "unsupported godunov configuration");
+    }
+#undef EB_LAUNCH
     return check_launch(Fused ? "step_fused" : "godunov");
 }
 
@@ -417,240 +458,307 @@ const char* eb_last_error(void)
     return g_last_error.c_str();
 }
 
-int eb_set_godunov_tile(int by, int kchunk)
+int eb_set_godunov_config(int variant, int by, int kchunk)
 {
-    if (by != 4 && by != 8 && by != 16) {
-        return fail(EB_ERR_UNSUPPORTED, "tile height must be 4, 8 or 16");
-    }
     if (kchunk < 1) {
         return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 1");
     }
+    if (!config_supported(variant, by)) {
+        return fail(EB_ERR_UNSUPPORTED, "unsupported (variant, tile height): (0, 2|4) or (1, 4|8|16)");
+    }
+    g_variant.store(variant);
     g_tile_by.store(by);
     g_tile_kchunk.store(kchunk);
     return EB_OK;
 }
 
-int eb_get_godunov_tile(int* by, int* kchunk)
+int eb_get_godunov_config(int* variant, int* by, int* kchunk)
 {
-    if (by == nullptr || kchunk == nullptr) {
+    if (variant == nullptr || by == nullptr || kchunk == nullptr) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null output");
     }
+    *variant = g_variant.load();
     *by = g_tile_by.load();
     *kchunk = g_tile_kchunk.load();
     return EB_OK;
 }
 
-#define EB_CHECK(expr)            \
-    do {                          \
-        int const rc_ = (expr);   \
-        if (rc_ != EB_OK) {       \
-            return rc_;           \
-        }                         \
+} // extern "C"
+
+namespace {
+
+#define EB_CHECK(expr)          \
+    do {                        \
+        int const rc_ = (expr); \
+        if (rc_ != EB_OK) {     \
+            return rc_;         \
+        }                       \
     } while (0)
 
-#define EB_DISPATCH_INDEX(bits, CALL32, CALL64) ((bits) == 32 ? (CALL32) : (CALL64))
+// Entry-point bodies, shared by the f64 and f32 symbols. Extents are checked
+// first; array pointers may be null only when there is no cell to touch.
 
-#define EB_DEFINE_ABI(SUF, T)                                                                                     \
-    int eb_cons_to_prim_##SUF(hipStream_t stream, const eb_cons_##SUF* cons, const eb_prim_##SUF* prim,          \
-                              int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma)                        \
-    {                                                                                                             \
-        if (cons_null(cons) || prim_null(prim)) {                                                                 \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits, (run_cons_to_prim<T, int32_t>(stream, cons, prim, n0, n1, n2, gamma)), \
-                                 (run_cons_to_prim<T, int64_t>(stream, cons, prim, n0, n1, n2, gamma)));          \
-    }                                                                                                             \
-    int eb_prim_to_cons_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons,          \
-                              int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma)                        \
-    {                                                                                                             \
-        if (cons_null(cons) || prim_null(prim)) {                                                                 \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits, (run_prim_to_cons<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma)), \
-                                 (run_prim_to_cons<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma)));          \
-    }                                                                                                             \
-    int eb_time_step_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0,           \
-                           int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T* dt_dev)      \
-    {                                                                                                             \
-        if (ctx == nullptr || prim_null(prim) || dt_dev == nullptr) {                                             \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(                                                                                 \
-                index_bits,                                                                                       \
-                (run_time_step<T, int32_t>(ctx, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)),         \
-                (run_time_step<T, int64_t>(ctx, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)));        \
-    }                                                                                                             \
-    int eb_time_step_sync_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0,      \
-                                int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2,             \
-                                T* dt_host)                                                                       \
-    {                                                                                                             \
-        if (dt_host == nullptr) {                                                                                 \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null dt_host");                                                 \
-        }                                                                                                         \
-        if (ctx == nullptr) {                                                                                     \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null ctx");                                                     \
-        }                                                                                                         \
-        T* const dt_dev = reinterpret_cast<T*>(ctx->dt_slot);                                                     \
-        EB_CHECK(eb_time_step_##SUF(ctx, stream, prim, n0, n1, n2, index_bits, gamma, dx0, dx1, dx2, dt_dev));   \
-        if (hipMemcpyAsync(dt_host, dt_dev, sizeof(T), hipMemcpyDeviceToHost, stream) != hipSuccess               \
-            || hipStreamSynchronize(stream) != hipSuccess) {                                                      \
-            return fail(EB_ERR_HIP, "dt copy to host failed");                                                    \
-        }                                                                                                         \
-        return EB_OK;                                                                                             \
-    }                                                                                                             \
-    int eb_godunov_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons, int64_t n0,   \
-                         int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T dt)              \
-    {                                                                                                             \
-        if (cons_null(cons) || prim_null(prim)) {                                                                 \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits,                                                                      \
-                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          dt, nullptr, T(0), false)),                             \
-                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          dt, nullptr, T(0), false)));                            \
-    }                                                                                                             \
-    int eb_godunov_dev_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, const eb_cons_##SUF* cons,           \
-                             int64_t n0, int64_t n1, int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2,    \
-                             const T* dt_dev, T cfl)                                                              \
-    {                                                                                                             \
-        if (cons_null(cons) || prim_null(prim) || dt_dev == nullptr) {                                            \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits,                                                                      \
-                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          T(0), dt_dev, cfl, false)),                             \
-                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          T(0), dt_dev, cfl, false)));                            \
-    }                                                                                                             \
-    int eb_godunov_reference_order_##SUF(hipStream_t stream, const eb_prim_##SUF* prim,                          \
-                                         const eb_cons_##SUF* cons, int64_t n0, int64_t n1, int64_t n2,           \
-                                         int index_bits, T gamma, T dx0, T dx1, T dx2, T dt)                      \
-    {                                                                                                             \
-        if (cons_null(cons) || prim_null(prim)) {                                                                 \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits,                                                                      \
-                                 (run_godunov<T, int32_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          dt, nullptr, T(0), true)),                              \
-                                 (run_godunov<T, int64_t>(stream, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2,   \
-                                                          dt, nullptr, T(0), true)));                             \
-    }                                                                                                             \
-    int eb_ghost_refresh_##SUF(eb_context* ctx, hipStream_t stream, const eb_cons_##SUF* cons,                   \
-                               const eb_prim_##SUF* prim_out, int64_t n0, int64_t n1, int64_t n2,                 \
-                               int index_bits, T gamma, T dx0, T dx1, T dx2, T* ghost_min_dev)                    \
-    {                                                                                                             \
-        if (ctx == nullptr || cons_null(cons) || prim_null(prim_out) || ghost_min_dev == nullptr) {               \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        return EB_DISPATCH_INDEX(index_bits,                                                                      \
-                                 (run_ghost_refresh<T, int32_t>(ctx, stream, cons, prim_out, n0, n1, n2, gamma,   \
-                                                                dx0, dx1, dx2, ghost_min_dev)),                   \
-                                 (run_ghost_refresh<T, int64_t>(ctx, stream, cons, prim_out, n0, n1, n2, gamma,   \
-                                                                dx0, dx1, dx2, ghost_min_dev)));                  \
-    }                                                                                                             \
-    int eb_step_fused_##SUF(eb_context* ctx, hipStream_t stream, const eb_prim_##SUF* prim_in,                   \
-                            const eb_cons_##SUF* cons, const eb_prim_##SUF* prim_out, int64_t n0, int64_t n1,    \
-                            int64_t n2, int index_bits, T gamma, T dx0, T dx1, T dx2, T cfl, const T* dt_in,     \
-                            const T* ghost_min, T* dt_out, double* time_acc)                                      \
-    {                                                                                                             \
-        if (ctx == nullptr || prim_null(prim_in) || cons_null(cons) || prim_null(prim_out) || dt_in == nullptr    \
-            || dt_out == nullptr) {                                                                               \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null argument");                                                \
-        }                                                                                                         \
-        if (prim_in->d == prim_out->d || dt_in == dt_out) {                                                       \
-            return fail(EB_ERR_INVALID_ARGUMENT, "prim_out / dt_out must not alias the inputs");                  \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        if (n0 < 3 || n1 < 3 || n2 < 3) {                                                                         \
-            return fail(EB_ERR_INVALID_ARGUMENT, "fused step needs at least one interior cell");                  \
-        }                                                                                                         \
-        return EB_DISPATCH_INDEX(index_bits,                                                                      \
-                                 (run_step_fused<T, int32_t>(ctx, stream, prim_in, cons, prim_out, n0, n1, n2,    \
-                                                             gamma, dx0, dx1, dx2, cfl, dt_in, ghost_min, dt_out, \
-                                                             time_acc)),                                          \
-                                 (run_step_fused<T, int64_t>(ctx, stream, prim_in, cons, prim_out, n0, n1, n2,    \
-                                                             gamma, dx0, dx1, dx2, cfl, dt_in, ghost_min, dt_out, \
-                                                             time_acc)));                                         \
-    }                                                                                                             \
-    int eb_init_implode_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0, int64_t n1, int64_t n2, \
-                              int index_bits, T dx0, T dx1, T dx2)                                                \
-    {                                                                                                             \
-        if (prim_null(prim)) {                                                                                    \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        int64_t const n = n0 * n1 * n2;                                                                           \
-        if (n == 0) {                                                                                             \
-            return EB_OK;                                                                                         \
-        }                                                                                                         \
-        auto const p = prim_ptrs<T>(prim);                                                                        \
-        if (index_bits == 32) {                                                                                   \
-            hipLaunchKernelGGL((eb::k_init_implode<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,   \
-                               p, int32_t(n0), int32_t(n1), int32_t(n2), dx0, dx1, dx2);                          \
-        } else {                                                                                                  \
-            hipLaunchKernelGGL((eb::k_init_implode<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,   \
-                               p, n0, n1, n2, dx0, dx1, dx2);                                                     \
-        }                                                                                                         \
-        return check_launch("init_implode");                                                                      \
-    }                                                                                                             \
-    int eb_init_diagonal_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n0, int64_t n1,            \
-                               int64_t n2, int index_bits, T dx0, T dx1, T dx2, T offset, T d_hi, T p_hi,         \
-                               T d_lo, T p_lo)                                                                    \
-    {                                                                                                             \
-        if (prim_null(prim)) {                                                                                    \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n0, n1, n2, index_bits));                                                          \
-        int64_t const n = n0 * n1 * n2;                                                                           \
-        if (n == 0) {                                                                                             \
-            return EB_OK;                                                                                         \
-        }                                                                                                         \
-        auto const p = prim_ptrs<T>(prim);                                                                        \
-        if (index_bits == 32) {                                                                                   \
-            hipLaunchKernelGGL((eb::k_init_diagonal<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,  \
-                               p, int32_t(n0), int32_t(n1), int32_t(n2), dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, \
-                               p_lo);                                                                             \
-        } else {                                                                                                  \
-            hipLaunchKernelGGL((eb::k_init_diagonal<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream,  \
-                               p, n0, n1, n2, dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo);                     \
-        }                                                                                                         \
-        return check_launch("init_diagonal");                                                                     \
-    }                                                                                                             \
-    int eb_init_random_##SUF(hipStream_t stream, const eb_prim_##SUF* prim, int64_t n, int index_bits,           \
-                             uint64_t seed, double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma)   \
-    {                                                                                                             \
-        if (prim_null(prim)) {                                                                                    \
-            return fail(EB_ERR_INVALID_ARGUMENT, "null array");                                                   \
-        }                                                                                                         \
-        EB_CHECK(check_extents(n, 1, 1, index_bits));                                                             \
-        if (n == 0) {                                                                                             \
-            return EB_OK;                                                                                         \
-        }                                                                                                         \
-        auto const p = prim_ptrs<T>(prim);                                                                        \
-        if (index_bits == 32) {                                                                                   \
-            hipLaunchKernelGGL((eb::k_init_random<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream, p, \
-                               int32_t(n), static_cast<unsigned long long>(seed), d_lo, d_hi, p_lo, p_hi,         \
-                               u_sigma);                                                                          \
-        } else {                                                                                                  \
-            hipLaunchKernelGGL((eb::k_init_random<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, stream, p, \
-                               n, static_cast<unsigned long long>(seed), d_lo, d_hi, p_lo, p_hi, u_sigma);        \
-        }                                                                                                         \
-        return check_launch("init_random");                                                                       \
+template <class T, class CS, class PS>
+int abi_cons_to_prim(hipStream_t s, CS const* cons, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits,
+                     T gamma)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    return bits == 32 ? run_cons_to_prim<T, int32_t>(s, cons, prim, n0, n1, n2, gamma)
+                      : run_cons_to_prim<T, int64_t>(s, cons, prim, n0, n1, n2, gamma);
+}
+
+template <class T, class PS, class CS>
+int abi_prim_to_cons(hipStream_t s, PS const* prim, CS const* cons, int64_t n0, int64_t n1, int64_t n2, int bits,
+                     T gamma)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    return bits == 32 ? run_prim_to_cons<T, int32_t>(s, prim, cons, n0, n1, n2, gamma)
+                      : run_prim_to_cons<T, int64_t>(s, prim, cons, n0, n1, n2, gamma);
+}
+
+template <class T, class PS>
+int abi_time_step(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits,
+                  T gamma, T dx0, T dx1, T dx2, T* dt_dev)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (ctx == nullptr || dt_dev == nullptr || (n0 * n1 * n2 > 0 && prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    return bits == 32 ? run_time_step<T, int32_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)
+                      : run_time_step<T, int64_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev);
+}
+
+template <class T, class PS>
+int abi_time_step_sync(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits,
+                       T gamma, T dx0, T dx1, T dx2, T* dt_host)
+{
+    if (ctx == nullptr || dt_host == nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    T* const dt_dev = reinterpret_cast<T*>(ctx->dt_slot);
+    EB_CHECK(abi_time_step<T>(ctx, s, prim, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt_dev));
+    if (hipMemcpyAsync(dt_host, dt_dev, sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess
+        || hipStreamSynchronize(s) != hipSuccess) {
+        return fail(EB_ERR_HIP, "dt copy to host failed");
+    }
+    return EB_OK;
+}
+
+template <class T, class PS, class CS>
+int abi_godunov(hipStream_t s, PS const* prim, CS const* cons, int64_t n0, int64_t n1, int64_t n2, int bits,
+                T gamma, T dx0, T dx1, T dx2, T dt, T const* dt_dev, T cfl, bool reference_order)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    return bits == 32
+                   ? run_godunov<T, int32_t>(s, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2, dt, dt_dev, cfl,
+                                             reference_order)
+                   : run_godunov<T, int64_t>(s, prim, cons, n0, n1, n2, gamma, dx0, dx1, dx2, dt, dt_dev, cfl,
+                                             reference_order);
+}
+
+template <class T, class CS, class PS>
+int abi_ghost_refresh(eb_context* ctx, hipStream_t s, CS const* cons, PS const* prim_out, int64_t n0, int64_t n1,
+                      int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T* ghost_min)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (ctx == nullptr || ghost_min == nullptr || (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim_out)))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    return bits == 32 ? run_ghost_refresh<T, int32_t>(ctx, s, cons, prim_out, n0, n1, n2, gamma, dx0, dx1, dx2,
+                                                      ghost_min)
+                      : run_ghost_refresh<T, int64_t>(ctx, s, cons, prim_out, n0, n1, n2, gamma, dx0, dx1, dx2,
+                                                      ghost_min);
+}
+
+template <class T, class PS, class CS>
+int abi_step_fused(eb_context* ctx, hipStream_t s, PS const* prim_in, CS const* cons, PS const* prim_out, int64_t n0,
+                   int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T cfl, T const* dt_in,
+                   T const* ghost_min, T* dt_out, double* time_acc)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 < 3 || n1 < 3 || n2 < 3) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "the fused step needs at least one interior cell");
+    }
+    if (ctx == nullptr || prim_null(prim_in) || cons_null(cons) || prim_null(prim_out) || dt_in == nullptr
+        || dt_out == nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (prim_in->d == prim_out->d || dt_in == dt_out) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "prim_out / dt_out must not alias the inputs");
+    }
+    return bits == 32 ? run_step_fused<T, int32_t>(ctx, s, prim_in, cons, prim_out, n0, n1, n2, gamma, dx0, dx1,
+                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc)
+                      : run_step_fused<T, int64_t>(ctx, s, prim_in, cons, prim_out, n0, n1, n2, gamma, dx0, dx1,
+                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc);
+}
+
+template <class T, class I, class PS>
+void launch_implode(hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, T dx0, T dx1, T dx2)
+{
+    hipLaunchKernelGGL((eb::k_init_implode<T, I>), dim3(pointwise_grid(n0 * n1 * n2)), dim3(256), 0, s,
+                       prim_ptrs<T>(prim), static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), dx0, dx1,
+                       dx2);
+}
+
+template <class T, class PS>
+int abi_init_implode(hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits, T dx0, T dx1,
+                     T dx2)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 * n1 * n2 == 0) {
+        return EB_OK;
+    }
+    if (prim_null(prim)) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    if (bits == 32) {
+        launch_implode<T, int32_t>(s, prim, n0, n1, n2, dx0, dx1, dx2);
+    } else {
+        launch_implode<T, int64_t>(s, prim, n0, n1, n2, dx0, dx1, dx2);
+    }
+    return check_launch("init_implode");
+}
+
+template <class T, class I, class PS>
+void launch_diagonal(hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, T const* a)
+{
+    hipLaunchKernelGGL((eb::k_init_diagonal<T, I>), dim3(pointwise_grid(n0 * n1 * n2)), dim3(256), 0, s,
+                       prim_ptrs<T>(prim), static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), a[0], a[1],
+                       a[2], a[3], a[4], a[5], a[6], a[7]);
+}
+
+template <class T, class PS>
+int abi_init_diagonal(hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits, T dx0, T dx1,
+                      T dx2, T offset, T d_hi, T p_hi, T d_lo, T p_lo)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (n0 * n1 * n2 == 0) {
+        return EB_OK;
+    }
+    if (prim_null(prim)) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    T const a[8] = {dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo};
+    if (bits == 32) {
+        launch_diagonal<T, int32_t>(s, prim, n0, n1, n2, a);
+    } else {
+        launch_diagonal<T, int64_t>(s, prim, n0, n1, n2, a);
+    }
+    return check_launch("init_diagonal");
+}
+
+template <class T, class PS>
+int abi_init_random(hipStream_t s, PS const* prim, int64_t n, int bits, uint64_t seed, double d_lo, double d_hi,
+                    double p_lo, double p_hi, double u_sigma)
+{
+    EB_CHECK(check_extents(n, 1, 1, bits));
+    if (n == 0) {
+        return EB_OK;
+    }
+    if (prim_null(prim)) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    auto const p = prim_ptrs<T>(prim);
+    auto const sd = static_cast<unsigned long long>(seed);
+    if (bits == 32) {
+        hipLaunchKernelGGL((eb::k_init_random<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, p,
+                           static_cast<int32_t>(n), sd, d_lo, d_hi, p_lo, p_hi, u_sigma);
+    } else {
+        hipLaunchKernelGGL((eb::k_init_random<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, p, n, sd,
+                           d_lo, d_hi, p_lo, p_hi, u_sigma);
+    }
+    return check_launch("init_random");
+}
+
+#undef EB_CHECK
+
+} // namespace
+
+extern "C" {
+
+// ---- f64 / f32 symbols: thin forwarding wrappers --------------------------
+
+#define EB_ABI(SUF, T)                                                                                               \
+    int eb_cons_to_prim_##SUF(hipStream_t s, const eb_cons_##SUF* c, const eb_prim_##SUF* p, int64_t n0,            \
+                              int64_t n1, int64_t n2, int bits, T gamma)                                            \
+    {                                                                                                                \
+        return abi_cons_to_prim<T>(s, c, p, n0, n1, n2, bits, gamma);                                               \
+    }                                                                                                                \
+    int eb_prim_to_cons_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* c, int64_t n0,            \
+                              int64_t n1, int64_t n2, int bits, T gamma)                                            \
+    {                                                                                                                \
+        return abi_prim_to_cons<T>(s, p, c, n0, n1, n2, bits, gamma);                                               \
+    }                                                                                                                \
+    int eb_time_step_##SUF(eb_context* ctx, hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1,          \
+                           int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T* dt_dev)                           \
+    {                                                                                                                \
+        return abi_time_step<T>(ctx, s, p, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt_dev);                         \
+    }                                                                                                                \
+    int eb_time_step_sync_##SUF(eb_context* ctx, hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1,     \
+                                int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T* dt_host)                     \
+    {                                                                                                                \
+        return abi_time_step_sync<T>(ctx, s, p, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt_host);                   \
+    }                                                                                                                \
+    int eb_godunov_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* c, int64_t n0, int64_t n1,     \
+                         int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T dt)                                  \
+    {                                                                                                                \
+        return abi_godunov<T>(s, p, c, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt, nullptr, T(0), false);           \
+    }                                                                                                                \
+    int eb_godunov_dev_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* c, int64_t n0, int64_t n1, \
+                             int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, const T* dt_dev, T cfl)            \
+    {                                                                                                                \
+        if (dt_dev == nullptr) {                                                                                     \
+            return fail(EB_ERR_INVALID_ARGUMENT, "null dt_dev");                                                     \
+        }                                                                                                            \
+        return abi_godunov<T>(s, p, c, n0, n1, n2, bits, gamma, dx0, dx1, dx2, T(0), dt_dev, cfl, false);           \
+    }                                                                                                                \
+    int eb_godunov_reference_order_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* c, int64_t n0, \
+                                         int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T dt)      \
+    {                                                                                                                \
+        return abi_godunov<T>(s, p, c, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt, nullptr, T(0), true);            \
+    }                                                                                                                \
+    int eb_ghost_refresh_##SUF(eb_context* ctx, hipStream_t s, const eb_cons_##SUF* c, const eb_prim_##SUF* p,      \
+                               int64_t n0, int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2,          \
+                               T* ghost_min)                                                                         \
+    {                                                                                                                \
+        return abi_ghost_refresh<T>(ctx, s, c, p, n0, n1, n2, bits, gamma, dx0, dx1, dx2, ghost_min);               \
+    }                                                                                                                \
+    int eb_step_fused_##SUF(eb_context* ctx, hipStream_t s, const eb_prim_##SUF* pin, const eb_cons_##SUF* c,       \
+                            const eb_prim_##SUF* pout, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma,       \
+                            T dx0, T dx1, T dx2, T cfl, const T* dt_in, const T* ghost_min, T* dt_out,              \
+                            double* time_acc)                                                                        \
+    {                                                                                                                \
+        return abi_step_fused<T>(ctx, s, pin, c, pout, n0, n1, n2, bits, gamma, dx0, dx1, dx2, cfl, dt_in,          \
+                                 ghost_min, dt_out, time_acc);                                                       \
+    }                                                                                                                \
+    int eb_init_implode_##SUF(hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1, int64_t n2, int bits,  \
+                              T dx0, T dx1, T dx2)                                                                   \
+    {                                                                                                                \
+        return abi_init_implode<T>(s, p, n0, n1, n2, bits, dx0, dx1, dx2);                                          \
+    }                                                                                                                \
+    int eb_init_diagonal_##SUF(hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1, int64_t n2, int bits, \
+                               T dx0, T dx1, T dx2, T offset, T d_hi, T p_hi, T d_lo, T p_lo)                        \
+    {                                                                                                                \
+        return abi_init_diagonal<T>(s, p, n0, n1, n2, bits, dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo);         \
+    }                                                                                                                \
+    int eb_init_random_##SUF(hipStream_t s, const eb_prim_##SUF* p, int64_t n, int bits, uint64_t seed,             \
+                             double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma)                     \
+    {                                                                                                                \
+        return abi_init_random<T>(s, p, n, bits, seed, d_lo, d_hi, p_lo, p_hi, u_sigma);                            \
     }
 
-EB_DEFINE_ABI(f64, double)
-EB_DEFINE_ABI(f32, float)
+EB_ABI(f64, double)
+EB_ABI(f32, float)
 
-#undef EB_DEFINE_ABI
-#undef EB_DISPATCH_INDEX
-#undef EB_CHECK
+#undef EB_ABI
 
 } // extern "C"

@@ -157,12 +157,21 @@ __device__ __forceinline__ T normal(Prim<T> const& q)
     }
 }
 
-// HLLC flux across a face normal to Dir, left state qL (sound speed cL), right
-// state qR (cR). Davis wave speeds, acoustic star state, upwind side chosen by
-// u* > 0 (ties to the right), supersonic test S_L*S_R > 0; the energy star term
-// divides by (S - u*). Follows hllc.hpp:61-103 line by line.
+// Total energy per volume p/(gamma-1) + rho|u|^2/2 in the order of hllc.hpp:82.
+template <class T>
+__device__ __forceinline__ T total_energy(Prim<T> const& q, T gm1)
+{
+    return q.p / gm1 + ke_prim(q);
+}
+
+// HLLC flux across a face normal to Dir, left state qL (sound speed cL, total
+// energy eL), right state qR (cR, eR). Davis wave speeds, acoustic star state,
+// upwind side chosen by u* > 0 (ties to the right), supersonic test
+// S_L*S_R > 0; the energy star term divides by (S - u*). Follows hllc.hpp:61-103
+// line by line; cL, cR, eL, eR are the per-cell values hllc.hpp:64-65,82 computes,
+// passed in so that a cell computes them once for all of its faces.
 template <int Dir, class T>
-__device__ __forceinline__ Flux<T> hllc_flux(Prim<T> const& qL, T cL, Prim<T> const& qR, T cR, T gm1)
+__device__ __forceinline__ Flux<T> hllc_flux(Prim<T> const& qL, T cL, T eL, Prim<T> const& qR, T cR, T eR)
 {
     T const unL = normal<Dir>(qL);
     T const unR = normal<Dir>(qR);
@@ -175,26 +184,23 @@ __device__ __forceinline__ Flux<T> hllc_flux(Prim<T> const& qL, T cL, Prim<T> co
     T const pstar = T(0.5) * (qL.p + qR.p + rcL * (ustar - unL) + rcR * (ustar - unR));
     bool const left = ustar > T(0);
     T const S = left ? SL : SR;
-    Prim<T> const q {left ? qL.d : qR.d,
-                     left ? qL.p : qR.p,
-                     left ? qL.u0 : qR.u0,
-                     left ? qL.u1 : qR.u1,
-                     left ? qL.u2 : qR.u2};
-    T const un = normal<Dir>(q);
-    T const etot = q.p / gm1 + ke_prim(q);
+    T const qd = left ? qL.d : qR.d;
+    T const qp = left ? qL.p : qR.p;
+    T const un = left ? unL : unR;
+    T const etot = left ? eL : eR;
     bool const supersonic = SL * SR > T(0);
     T const un_o = supersonic ? un : ustar;
-    T const ptot_o = supersonic ? q.p : pstar;
+    T const ptot_o = supersonic ? qp : pstar;
     T const ratio = (S - un) / (S - un_o);
-    T const d_o = ratio * q.d;
-    T const etot_o = (ratio * etot) + ((ptot_o * un_o - q.p * un) / (S - ustar));
+    T const d_o = ratio * qd;
+    T const etot_o = (ratio * etot) + ((ptot_o * un_o - qp * un) / (S - ustar));
     T const mass = d_o * un_o;
     Flux<T> f;
     f.d = mass;
     f.e = (etot_o + ptot_o) * un_o;
-    f.m0 = mass * q.u0;
-    f.m1 = mass * q.u1;
-    f.m2 = mass * q.u2;
+    f.m0 = mass * (left ? qL.u0 : qR.u0);
+    f.m1 = mass * (left ? qL.u1 : qR.u1);
+    f.m2 = mass * (left ? qL.u2 : qR.u2);
     if constexpr (Dir == 0) {
         f.m0 = (mass * un_o) + ptot_o;
     } else if constexpr (Dir == 1) {
@@ -203,6 +209,13 @@ __device__ __forceinline__ Flux<T> hllc_flux(Prim<T> const& qL, T cL, Prim<T> co
         f.m2 = (mass * un_o) + ptot_o;
     }
     return f;
+}
+
+// Same flux with the per-cell quantities computed here (reference-order kernel).
+template <int Dir, class T>
+__device__ __forceinline__ Flux<T> hllc_flux(Prim<T> const& qL, T cL, Prim<T> const& qR, T cR, T gm1)
+{
+    return hllc_flux<Dir>(qL, cL, total_energy(qL, gm1), qR, cR, total_energy(qR, gm1));
 }
 
 // flux += ds * (fR - fL), component-wise (godunov.hpp:58-62 and its y/z twins).

@@ -269,26 +269,111 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk)
     return (b & 7) * (nblk >> 3) + (b >> 3);
 }
 
-// Tiled, face-once Godunov update (2.5-D march).
+// Plane-relative addressing: a wave-uniform plane base pointer (scalar registers,
+// index type I, so 64-bit extents are honoured) plus a per-lane 32-bit byte offset
+// inside the plane. This is the saddr + voffset form of global loads/stores: one
+// VGPR of address per lane for all five fields.
+template <class T>
+__device__ __forceinline__ T ld(T const* plane, uint32_t byte_off)
+{
+    return *reinterpret_cast<T const*>(reinterpret_cast<char const*>(plane) + byte_off);
+}
+
+template <class T>
+__device__ __forceinline__ void st(T* plane, uint32_t byte_off, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + byte_off) = v;
+}
+
+template <class T, class I>
+__device__ __forceinline__ Prim<T> ld_prim(PrimPtrs<T const> const& a, I plane, uint32_t off)
+{
+    return {ld(a.d + plane, off), ld(a.p + plane, off), ld(a.ux0 + plane, off), ld(a.ux1 + plane, off),
+            ld(a.ux2 + plane, off)};
+}
+
+template <class T, class I>
+__device__ __forceinline__ Cons<T> ld_cons(ConsPtrs<T> const& a, I plane, uint32_t off)
+{
+    return {ld<T>(a.d + plane, off), ld<T>(a.e + plane, off), ld<T>(a.mx0 + plane, off), ld<T>(a.mx1 + plane, off),
+            ld<T>(a.mx2 + plane, off)};
+}
+
+template <class T, class I>
+__device__ __forceinline__ void st_cons(ConsPtrs<T> const& a, I plane, uint32_t off, Cons<T> const& u)
+{
+    st(a.d + plane, off, u.d);
+    st(a.e + plane, off, u.e);
+    st(a.mx0 + plane, off, u.m0);
+    st(a.mx1 + plane, off, u.m1);
+    st(a.mx2 + plane, off, u.m2);
+}
+
+template <class T, class I>
+__device__ __forceinline__ void st_prim(PrimPtrs<T> const& a, I plane, uint32_t off, Prim<T> const& q)
+{
+    st(a.d + plane, off, q.d);
+    st(a.p + plane, off, q.p);
+    st(a.ux0 + plane, off, q.u0);
+    st(a.ux1 + plane, off, q.u1);
+    st(a.ux2 + plane, off, q.u2);
+}
+
+// s_waitcnt vmcnt(0) through the builtin, so the compiler's own wait tracking
+// knows that every earlier vector-memory operation (LDS transfers included) has
+// completed. Encoding for gfx9-family targets: vmcnt 0, expcnt 7, lgkmcnt 15.
+// (Device-only builtins are kept out of the host pass, where a __global__
+// template's body is parsed but not compiled.)
+__device__ __forceinline__ void wait_vmem_all()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+#endif
+}
+
+// One 16-byte transfer per lane from global memory straight into LDS. The LDS
+// destination is wave-uniform: lane l writes lds + 16*l bytes.
+__device__ __forceinline__ void transfer16(void const* global, void* lds)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(global, lds, 16, 0, 0);
+#endif
+}
+
+// Workgroup barrier that orders LDS traffic only: it waits for this wave's LDS
+// operations, not for its outstanding global loads, so prefetches stay in flight
+// across it. No kernel here communicates through global memory inside a launch.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Tiled, face-once Godunov update (2.5-D march along k).
 //
 // A block of 64 x BY threads owns a 64 x BY tile of interior cells in (i, j) and
-// marches along k over `kchunk` planes. Per plane:
-//   A. every thread stages its own cell (primitive + sound speed) in LDS, and the
-//      first 128 + 2*BY threads stage the halo ring (rows j0-1, j0+BY; columns
-//      i0-1, i0+64);
-//   B. each thread solves the x face on its low side, the y face on its low side,
-//      and the z face on its high side; the tile's high-side x faces (BY of them)
-//      and high-side y faces (64) are solved by two of the waves;
-//   C. each thread combines low/high faces in the reference's order
-//      (x, then y, then z; godunov.hpp:48-82), updates cons in place.
-// The z face on the low side comes from the previous plane (registers), the x face
-// on the high side from the next lane (wave shuffle), the y face on the high side
-// from LDS. Every face is solved once, with the same operands as the reference's
-// second solve of it, so results are bitwise those of the reference order.
+// marches along k over `kchunk` planes. Per plane k:
+//   A. each thread stages its own cell (primitives, sound speed c and total energy
+//      E, hllc.hpp:64-65,82) in LDS, and the first 128 + 2*BY threads stage the halo
+//      ring (rows j0-1 and j0+BY, columns i0-1 and i0+64);
+//   B. each thread solves the x face and the y face on its low side and the z face
+//      on its high side; the tile's high-side x faces (BY of them) and y faces (64)
+//      are solved by two of the waves;
+//   C. each thread combines low and high faces in the reference's order (x, then y,
+//      then z; godunov.hpp:48-82) and updates its cell in place.
+// The low z face is carried from the previous plane in registers, the high x face
+// comes from the next lane (wave shuffle), the high y face from LDS. Each face is
+// solved once with the operands of the reference's second solve of it, so the
+// result is bitwise that of the reference order.
 //
-// Fused = true additionally converts the updated cell to primitives into
-// prim_out (cons_to_prim.hpp:28-31) and reduces the next dt over the new
-// primitives (time_step.hpp:28-38), publishing it through `sink`.
+// Loads of plane k (own cell of plane k+1, halo and conservatives of plane k) are
+// issued at the top of the plane; the barriers only order LDS, so the
+// conservatives stay in flight through the flux work.
+//
+// Fused = true additionally converts the updated cell to primitives into prim_out
+// (cons_to_prim.hpp:28-31) and reduces the next dt over the new primitives
+// (time_step.hpp:28-38), publishing it through `sink`.
+//
+// Requires n0, n1, n2 >= 3 and n0*n1*sizeof(T) < 2^32 (32-bit in-plane offsets).
 template <class T, class I, int BY, bool Fused>
 __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         PrimPtrs<T const> prim,
@@ -316,9 +401,10 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     constexpr int LX = BX + 2;
     constexpr int LY = BY + 2;
     constexpr int NHALO = 2 * BX + 2 * BY;
-    __shared__ T sq[6][LY][LX];   // staged plane: d, p, u0, u1, u2, c
+    constexpr int NF = 7; // staged per cell: d, p, u0, u1, u2, c, E
+    __shared__ T sq[NF][LY][LX];
     __shared__ T sfy[5][BY + 1][BX]; // y faces: row r sits between tile rows r-1 and r
-    __shared__ T sfx[5][BY];      // x faces on the tile's high-i edge
+    __shared__ T sfx[5][BY];         // x faces on the tile's high-i edge
 
     int const nblk = tiles_x * tiles_y * tiles_z;
     int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
@@ -333,22 +419,20 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     I const j0 = 1 + static_cast<I>(by) * BY;
     I const kbeg = 1 + static_cast<I>(bz) * kchunk;
     I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
-    I const s1 = n0;
     I const s2 = n0 * n1;
 
-    // Own cell (clamped into the array for loads; only valid cells are written).
+    // Own cell, clamped into the array for loads (only valid cells are written).
     I const i = i0 + tx;
     I const j = j0 + ty;
     bool const valid = (i <= n0 - 2) && (j <= n1 - 2);
     I const ic = (i < n0 - 1) ? i : n0 - 1;
     I const jc = (j < n1 - 1) ? j : n1 - 1;
-    I const col = ic + s1 * jc;
+    uint32_t const own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
 
-    // Halo cell handled by this thread (if any): LDS (row, column) and clamped
-    // global (i, j).
+    // Halo cell staged by this thread (if any): LDS (row, column), clamped offset.
     int hrow = 0;
     int hcol = 0;
-    I hcolg = 0;
+    uint32_t halo = 0;
     bool const has_halo = lid < NHALO;
     if (has_halo) {
         if (lid < BX) {
@@ -368,56 +452,58 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         I hj = j0 - 1 + hrow;
         hi = (hi < n0 - 1) ? hi : n0 - 1;
         hj = (hj < n1 - 1) ? hj : n1 - 1;
-        hcolg = hi + s1 * hj;
+        halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
     }
 
     T const dtodv = resolve_dtodv(dta);
     T local_min = Limits<T>::max();
 
-    // Prologue: z face below the first plane.
-    Prim<T> qc = load_prim(prim, col + s2 * kbeg);
-    T cc = sound_speed(gamma, qc.d, qc.p);
+    auto stage = [&](int r, int c, Prim<T> const& q, T cs, T e) {
+        sq[0][r][c] = q.d;
+        sq[1][r][c] = q.p;
+        sq[2][r][c] = q.u0;
+        sq[3][r][c] = q.u1;
+        sq[4][r][c] = q.u2;
+        sq[5][r][c] = cs;
+        sq[6][r][c] = e;
+    };
+    auto cell = [&](int r, int c) { return Prim<T> {sq[0][r][c], sq[1][r][c], sq[2][r][c], sq[3][r][c], sq[4][r][c]}; };
+
+    // Prologue: z face below the first plane, own plane kbeg in registers.
     Flux<T> fz_lo;
+    Prim<T> q_cur = ld_prim(prim, s2 * kbeg, own);
+    T c_cur = sound_speed(gamma, q_cur.d, q_cur.p);
+    T e_cur = total_energy(q_cur, gm1);
     {
-        Prim<T> const qm = load_prim(prim, col + s2 * (kbeg - 1));
-        T const cm = sound_speed(gamma, qm.d, qm.p);
-        fz_lo = hllc_flux<2>(qm, cm, qc, cc, gm1);
+        Prim<T> const qm = ld_prim(prim, s2 * (kbeg - 1), own);
+        fz_lo = hllc_flux<2>(qm, sound_speed(gamma, qm.d, qm.p), total_energy(qm, gm1), q_cur, c_cur, e_cur);
     }
-    Prim<T> qn = load_prim(prim, col + s2 * (kbeg + 1));
-    T cn = sound_speed(gamma, qn.d, qn.p);
+    Prim<T> q_next {};
+    Prim<T> h_cur {};
 
     for (I k = kbeg; k < kend; ++k) {
-        // A. stage plane k.
-        sq[0][ty + 1][tx + 1] = qc.d;
-        sq[1][ty + 1][tx + 1] = qc.p;
-        sq[2][ty + 1][tx + 1] = qc.u0;
-        sq[3][ty + 1][tx + 1] = qc.u1;
-        sq[4][ty + 1][tx + 1] = qc.u2;
-        sq[5][ty + 1][tx + 1] = cc;
+        I const plane = s2 * k;
+        q_next = ld_prim(prim, plane + s2, own);
         if (has_halo) {
-            Prim<T> const h = load_prim(prim, hcolg + s2 * k);
-            sq[0][hrow][hcol] = h.d;
-            sq[1][hrow][hcol] = h.p;
-            sq[2][hrow][hcol] = h.u0;
-            sq[3][hrow][hcol] = h.u1;
-            sq[4][hrow][hcol] = h.u2;
-            sq[5][hrow][hcol] = sound_speed(gamma, h.d, h.p);
+            h_cur = ld_prim(prim, plane, halo);
         }
-        Cons<T> u = load_cons(cons, col + s2 * k);
-        __syncthreads();
+        Cons<T> u = ld_cons(cons, plane, own);
+
+        // A. stage plane k.
+        stage(ty + 1, tx + 1, q_cur, c_cur, e_cur);
+        if (has_halo) {
+            stage(hrow, hcol, h_cur, sound_speed(gamma, h_cur.d, h_cur.p), total_energy(h_cur, gm1));
+        }
+        lds_barrier();
 
         // B. faces.
-        Flux<T> fx_lo;
-        Flux<T> fy_lo;
-        {
-            Prim<T> const qw {sq[0][ty + 1][tx], sq[1][ty + 1][tx], sq[2][ty + 1][tx], sq[3][ty + 1][tx], sq[4][ty + 1][tx]};
-            fx_lo = hllc_flux<0>(qw, sq[5][ty + 1][tx], qc, cc, gm1);
-        }
-        {
-            Prim<T> const qs {sq[0][ty][tx + 1], sq[1][ty][tx + 1], sq[2][ty][tx + 1], sq[3][ty][tx + 1], sq[4][ty][tx + 1]};
-            fy_lo = hllc_flux<1>(qs, sq[5][ty][tx + 1], qc, cc, gm1);
-        }
-        Flux<T> const fz_hi = hllc_flux<2>(qc, cc, qn, cn, gm1);
+        T const c_next = sound_speed(gamma, q_next.d, q_next.p);
+        T const e_next = total_energy(q_next, gm1);
+        Flux<T> const fx_lo
+                = hllc_flux<0>(cell(ty + 1, tx), sq[5][ty + 1][tx], sq[6][ty + 1][tx], q_cur, c_cur, e_cur);
+        Flux<T> const fy_lo
+                = hllc_flux<1>(cell(ty, tx + 1), sq[5][ty][tx + 1], sq[6][ty][tx + 1], q_cur, c_cur, e_cur);
+        Flux<T> const fz_hi = hllc_flux<2>(q_cur, c_cur, e_cur, q_next, c_next, e_next);
         sfy[0][ty][tx] = fy_lo.d;
         sfy[1][ty][tx] = fy_lo.e;
         sfy[2][ty][tx] = fy_lo.m0;
@@ -425,9 +511,8 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         sfy[4][ty][tx] = fy_lo.m2;
         if (ty == BY - 1) {
             // High-side y faces of the tile: rows BY and BY+1 of the staged plane.
-            Prim<T> const a {sq[0][BY][tx + 1], sq[1][BY][tx + 1], sq[2][BY][tx + 1], sq[3][BY][tx + 1], sq[4][BY][tx + 1]};
-            Prim<T> const b {sq[0][BY + 1][tx + 1], sq[1][BY + 1][tx + 1], sq[2][BY + 1][tx + 1], sq[3][BY + 1][tx + 1], sq[4][BY + 1][tx + 1]};
-            Flux<T> const f = hllc_flux<1>(a, sq[5][BY][tx + 1], b, sq[5][BY + 1][tx + 1], gm1);
+            Flux<T> const f = hllc_flux<1>(cell(BY, tx + 1), sq[5][BY][tx + 1], sq[6][BY][tx + 1],
+                                           cell(BY + 1, tx + 1), sq[5][BY + 1][tx + 1], sq[6][BY + 1][tx + 1]);
             sfy[0][BY][tx] = f.d;
             sfy[1][BY][tx] = f.e;
             sfy[2][BY][tx] = f.m0;
@@ -435,18 +520,17 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
             sfy[4][BY][tx] = f.m2;
         }
         if (ty == (BY > 1 ? BY - 2 : 0) && tx < BY) {
-            // High-side x faces of the tile: columns 64 and 65 of row tx.
+            // High-side x faces of the tile: columns 64 and 65 of row tx+1.
             int const r = tx + 1;
-            Prim<T> const a {sq[0][r][BX], sq[1][r][BX], sq[2][r][BX], sq[3][r][BX], sq[4][r][BX]};
-            Prim<T> const b {sq[0][r][BX + 1], sq[1][r][BX + 1], sq[2][r][BX + 1], sq[3][r][BX + 1], sq[4][r][BX + 1]};
-            Flux<T> const f = hllc_flux<0>(a, sq[5][r][BX], b, sq[5][r][BX + 1], gm1);
+            Flux<T> const f = hllc_flux<0>(cell(r, BX), sq[5][r][BX], sq[6][r][BX], cell(r, BX + 1),
+                                           sq[5][r][BX + 1], sq[6][r][BX + 1]);
             sfx[0][tx] = f.d;
             sfx[1][tx] = f.e;
             sfx[2][tx] = f.m0;
             sfx[3][tx] = f.m1;
             sfx[4][tx] = f.m2;
         }
-        __syncthreads();
+        lds_barrier();
 
         // C. combine and update.
         Flux<T> fx_hi;
@@ -458,18 +542,18 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         if (tx == BX - 1) {
             fx_hi = Flux<T> {sfx[0][ty], sfx[1][ty], sfx[2][ty], sfx[3][ty], sfx[4][ty]};
         }
-        Flux<T> const fy_hi {sfy[0][ty + 1][tx], sfy[1][ty + 1][tx], sfy[2][ty + 1][tx], sfy[3][ty + 1][tx], sfy[4][ty + 1][tx]};
+        Flux<T> const fy_hi {sfy[0][ty + 1][tx], sfy[1][ty + 1][tx], sfy[2][ty + 1][tx], sfy[3][ty + 1][tx],
+                             sfy[4][ty + 1][tx]};
         Flux<T> acc {T(0), T(0), T(0), T(0), T(0)};
         accumulate(acc, ds0, fx_hi, fx_lo);
         accumulate(acc, ds1, fy_hi, fy_lo);
         accumulate(acc, ds2, fz_hi, fz_lo);
         apply_update(u, dtodv, acc);
         if (valid) {
-            I const c = col + s2 * k;
-            store_cons(cons, c, u);
+            st_cons(cons, plane, own, u);
             if constexpr (Fused) {
                 Prim<T> const q1 = cons_to_prim_pt(u, gm1);
-                store_prim(prim_out, c, q1);
+                st_prim(prim_out, plane, own, q1);
                 T const c1 = sound_speed(gamma, q1.d, q1.p);
                 local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
             }
@@ -477,16 +561,514 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
 
         // Advance the march.
         fz_lo = fz_hi;
-        qc = qn;
-        cc = cn;
-        if (k + 1 < kend) {
-            qn = load_prim(prim, col + s2 * (k + 2));
-            cn = sound_speed(gamma, qn.d, qn.p);
-        }
+        q_cur = q_next;
+        c_cur = c_next;
+        e_cur = e_next;
     }
 
     if constexpr (Fused) {
         T const bm = block_min<T, BY>(local_min);
+        if (lid == 0) {
+            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+        }
+    }
+}
+
+// Streamed Godunov update (2.5-D march along k) with a producer wave.
+//
+// A block owns a 64 x BY tile of interior cells in (i, j), marches along k over
+// `kchunk` planes, and has BY compute waves (one per tile row, one thread per
+// cell) plus one producer wave. The producer streams the primitive fields of
+// plane k+2, halo ring included, straight from HBM into LDS with 16-byte
+// global_load_lds transfers into a three-plane ring, while the compute waves work
+// on plane k; the compute waves never touch global memory for primitives, so they
+// never wait on those transfers, and the producer waits for one plane's transfers
+// a full plane after issuing them.
+//
+// Per plane k (three LDS barriers):
+//   0. plane k+1 has landed (the producer waited before arriving);
+//   A. compute waves load the conservatives of plane k into registers and compute
+//      the sound speed c and total energy E (hllc.hpp:64-65,82) of every tile cell
+//      of plane k+1; the producer issues plane k+2;
+//   B. faces of plane k: x and y faces on each cell's low side, z face on its high
+//      side; the tile's high-side x faces (BY) and y faces (64) are solved by two of
+//      the waves;
+//   C. each thread combines faces in the reference's order (x, y, z;
+//      godunov.hpp:48-82) and updates its cell in place.
+// Every face is solved once with the operands of the reference's second solve of
+// it, so results are bitwise those of the reference order.
+//
+// LDS rows are 16-byte aligned copies of global rows: row r of a plane holds
+// elements starting at the aligned-down linear index of cell (i0-1, j0-1+r), so a
+// cell's column is shifted by that row's misalignment `shift`.
+//
+// Requires n0, n1, n2 >= 3, n0*n1*sizeof(T) < 2^32 and 16-byte aligned fields.
+template <class T, class I, int BY, bool Fused>
+__global__ __launch_bounds__(64 * (BY + 1)) void k_godunov_stream(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    constexpr int BX = 64;
+    constexpr int LX = BX + 2;
+    constexpr int LY = BY + 2;
+    constexpr int E = 16 / static_cast<int>(sizeof(T)); // elements per 16-byte transfer
+    constexpr int P = ((LX + E - 1) + E - 1) / E * E;   // LDS row pitch with room for the shift
+    constexpr int L = P / E;                            // transfers per row
+    constexpr int NXFER = LY * L;                       // transfers per field and plane
+    constexpr int NCH = (NXFER + 63) / 64;              // wave instructions per field and plane
+    constexpr int NCELL = LY * LX;
+    __shared__ __align__(16) T sp[3][5][LY * P]; // primitive ring: planes k-1+slot
+    __shared__ T sd[2][2][LY][LX];               // c and E of planes k and k+1
+    __shared__ T sfy[5][BY + 1][BX];
+    __shared__ T sfx[5][BY];
+
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    int const bx = t % tiles_x;
+    int const by = (t / tiles_x) % tiles_y;
+    int const bz = t / (tiles_x * tiles_y);
+    int const tx = threadIdx.x;
+    int const ty = threadIdx.y;
+    bool const producer = ty == BY;
+
+    I const i0 = 1 + static_cast<I>(bx) * BX;
+    I const j0 = 1 + static_cast<I>(by) * BY;
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    I const s2 = n0 * n1;
+    I const ntot = s2 * n2;
+
+    // Clamped global row of LDS row r, and the 32-bit pieces of its linear index
+    // used to find the row's misalignment in plane k.
+    auto row_j = [&](int r) {
+        I const jr = j0 - 1 + r;
+        return (jr < n1 - 1) ? jr : n1 - 1;
+    };
+    auto shift = [&](int r, I k) {
+        uint32_t const lin = static_cast<uint32_t>(i0 - 1) + static_cast<uint32_t>(n0) * static_cast<uint32_t>(row_j(r))
+                             + static_cast<uint32_t>(s2) * static_cast<uint32_t>(k);
+        return static_cast<int>(lin & static_cast<uint32_t>(E - 1));
+    };
+    auto slot = [&](I k) { return static_cast<int>((k - kbeg + 1) % 3); };
+    auto dslot = [&](I k) { return static_cast<int>((k - kbeg + 1) & 1); };
+
+    // Producer: all transfers of plane k into ring slot b.
+    auto transfer = [&](I k, int b) {
+        T const* const fields[5] = {prim.d, prim.p, prim.ux0, prim.ux1, prim.ux2};
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            int const x = ch * 64 + tx;
+            if (x < NXFER) {
+                int const r = x / L;
+                int const seg = x % L;
+                I const rowlin = (i0 - 1) + n0 * (row_j(r) + n1 * k);
+                I const g = rowlin - (rowlin & static_cast<I>(E - 1)) + static_cast<I>(seg * E);
+                if (g + E <= ntot) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) {
+                        transfer16(fields[f] + g, &sp[b][f][ch * 64 * E]);
+                    }
+                }
+            }
+        }
+    };
+
+    // Compute-wave views of the ring.
+    auto prim_at = [&](int b, int r, int c, int sh) {
+        int const o = r * P + sh + c;
+        return Prim<T> {sp[b][0][o], sp[b][1][o], sp[b][2][o], sp[b][3][o], sp[b][4][o]};
+    };
+    // c and E of every tile cell of plane k (ring slot b) into sd[db].
+    auto derive = [&](I k, int b, int db) {
+        for (int n = tx + BX * ty; n < NCELL; n += BX * BY) {
+            int const r = n / LX;
+            int const c = n % LX;
+            Prim<T> const q = prim_at(b, r, c, shift(r, k));
+            sd[db][0][r][c] = sound_speed(gamma, q.d, q.p);
+            sd[db][1][r][c] = total_energy(q, gm1);
+        }
+    };
+
+    T const dtodv = resolve_dtodv(dta);
+    T local_min = Limits<T>::max();
+
+    // Own cell (compute waves), clamped into the array for loads.
+    I const i = i0 + tx;
+    I const j = j0 + ty;
+    bool const valid = !producer && (i <= n0 - 2) && (j <= n1 - 2);
+    I const ic = (i < n0 - 1) ? i : n0 - 1;
+    I const jc = (j < n1 - 1) ? j : n1 - 1;
+    uint32_t const own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
+
+    // Prologue: planes kbeg-1, kbeg, kbeg+1 in the ring; c, E of plane kbeg for the
+    // tile; the low z face of plane kbeg in registers.
+    if (producer) {
+        transfer(kbeg - 1, slot(kbeg - 1));
+        transfer(kbeg, slot(kbeg));
+        transfer(kbeg + 1, slot(kbeg + 1));
+        wait_vmem_all();
+    }
+    lds_barrier();
+    Flux<T> fz_lo {};
+    if (!producer) {
+        derive(kbeg, slot(kbeg), dslot(kbeg));
+        Prim<T> const qm = prim_at(slot(kbeg - 1), ty + 1, tx + 1, shift(ty + 1, kbeg - 1));
+        Prim<T> const q0 = prim_at(slot(kbeg), ty + 1, tx + 1, shift(ty + 1, kbeg));
+        fz_lo = hllc_flux<2>(qm, sound_speed(gamma, qm.d, qm.p), total_energy(qm, gm1), q0,
+                             sound_speed(gamma, q0.d, q0.p), total_energy(q0, gm1));
+    }
+    lds_barrier();
+
+    if (producer) {
+        for (I k = kbeg; k < kend; ++k) {
+            // 0. plane k+1 landed (issued a plane ago); then stream plane k+2 into
+            // the slot of plane k-1, which no wave reads any more.
+            wait_vmem_all();
+            lds_barrier();
+            if (k + 2 <= kend) {
+                transfer(k + 2, slot(k + 2));
+            }
+            lds_barrier(); // A | B
+            lds_barrier(); // B | C
+        }
+    } else {
+        for (I k = kbeg; k < kend; ++k) {
+            I const plane = s2 * k;
+            int const b = slot(k);
+            int const bn = slot(k + 1);
+            int const db = dslot(k);
+            int const dbn = dslot(k + 1);
+            lds_barrier(); // 0
+
+            // A. conservatives of plane k; c, E of plane k+1.
+            Cons<T> u = ld_cons(cons, plane, own);
+            derive(k + 1, bn, dbn);
+            lds_barrier();
+
+            // B. faces.
+            int const sh_own = shift(ty + 1, k);
+            int const sh_dn = shift(ty, k);
+            Prim<T> const qc = prim_at(b, ty + 1, tx + 1, sh_own);
+            T const cc = sd[db][0][ty + 1][tx + 1];
+            T const ec = sd[db][1][ty + 1][tx + 1];
+            Flux<T> const fx_lo = hllc_flux<0>(prim_at(b, ty + 1, tx, sh_own), sd[db][0][ty + 1][tx],
+                                               sd[db][1][ty + 1][tx], qc, cc, ec);
+            Flux<T> const fy_lo = hllc_flux<1>(prim_at(b, ty, tx + 1, sh_dn), sd[db][0][ty][tx + 1],
+                                               sd[db][1][ty][tx + 1], qc, cc, ec);
+            Flux<T> const fz_hi = hllc_flux<2>(qc, cc, ec, prim_at(bn, ty + 1, tx + 1, shift(ty + 1, k + 1)),
+                                               sd[dbn][0][ty + 1][tx + 1], sd[dbn][1][ty + 1][tx + 1]);
+            sfy[0][ty][tx] = fy_lo.d;
+            sfy[1][ty][tx] = fy_lo.e;
+            sfy[2][ty][tx] = fy_lo.m0;
+            sfy[3][ty][tx] = fy_lo.m1;
+            sfy[4][ty][tx] = fy_lo.m2;
+            if (ty == BY - 1) {
+                // High-side y faces of the tile: rows BY and BY+1.
+                int const s0 = shift(BY, k);
+                int const s1 = shift(BY + 1, k);
+                Flux<T> const f = hllc_flux<1>(prim_at(b, BY, tx + 1, s0), sd[db][0][BY][tx + 1], sd[db][1][BY][tx + 1],
+                                               prim_at(b, BY + 1, tx + 1, s1), sd[db][0][BY + 1][tx + 1],
+                                               sd[db][1][BY + 1][tx + 1]);
+                sfy[0][BY][tx] = f.d;
+                sfy[1][BY][tx] = f.e;
+                sfy[2][BY][tx] = f.m0;
+                sfy[3][BY][tx] = f.m1;
+                sfy[4][BY][tx] = f.m2;
+            }
+            if (ty == (BY > 1 ? BY - 2 : 0) && tx < BY) {
+                // High-side x faces of the tile: columns 64 and 65 of row tx+1.
+                int const r = tx + 1;
+                int const sr = shift(r, k);
+                Flux<T> const f = hllc_flux<0>(prim_at(b, r, BX, sr), sd[db][0][r][BX], sd[db][1][r][BX],
+                                               prim_at(b, r, BX + 1, sr), sd[db][0][r][BX + 1], sd[db][1][r][BX + 1]);
+                sfx[0][tx] = f.d;
+                sfx[1][tx] = f.e;
+                sfx[2][tx] = f.m0;
+                sfx[3][tx] = f.m1;
+                sfx[4][tx] = f.m2;
+            }
+            lds_barrier();
+
+            // C. combine and update.
+            Flux<T> fx_hi;
+            fx_hi.d = __shfl_down(fx_lo.d, 1, 64);
+            fx_hi.e = __shfl_down(fx_lo.e, 1, 64);
+            fx_hi.m0 = __shfl_down(fx_lo.m0, 1, 64);
+            fx_hi.m1 = __shfl_down(fx_lo.m1, 1, 64);
+            fx_hi.m2 = __shfl_down(fx_lo.m2, 1, 64);
+            if (tx == BX - 1) {
+                fx_hi = Flux<T> {sfx[0][ty], sfx[1][ty], sfx[2][ty], sfx[3][ty], sfx[4][ty]};
+            }
+            Flux<T> const fy_hi {sfy[0][ty + 1][tx], sfy[1][ty + 1][tx], sfy[2][ty + 1][tx], sfy[3][ty + 1][tx],
+                                 sfy[4][ty + 1][tx]};
+            Flux<T> acc {T(0), T(0), T(0), T(0), T(0)};
+            accumulate(acc, ds0, fx_hi, fx_lo);
+            accumulate(acc, ds1, fy_hi, fy_lo);
+            accumulate(acc, ds2, fz_hi, fz_lo);
+            apply_update(u, dtodv, acc);
+            if (valid) {
+                st_cons(cons, plane, own, u);
+                if constexpr (Fused) {
+                    Prim<T> const q1 = cons_to_prim_pt(u, gm1);
+                    st_prim(prim_out, plane, own, q1);
+                    T const c1 = sound_speed(gamma, q1.d, q1.p);
+                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                }
+            }
+            fz_lo = fz_hi;
+        }
+    }
+
+    if constexpr (Fused) {
+        T const bm = block_min<T, BY + 1>(local_min);
+        if (tx == 0 && ty == 0) {
+            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+        }
+    }
+}
+
+// Face-parallel Godunov update (2.5-D march along k), the default kernel.
+//
+// A block owns a 64 x BY tile of interior cells in (i, j) and marches along k over
+// `kchunk` planes. It has 3*BY waves in three roles, one wave per tile row each:
+//   X waves solve the x face on the low side of every cell of their row,
+//   Y waves solve the y face on the low side,
+//   Z waves solve the z face on the high side and update the cell.
+// The tile's high-side x faces (BY) are solved by X wave 0 and its high-side y
+// faces (64) by Y wave BY-1, in a second pass. Each thread therefore carries the
+// state of about one Riemann solve, which keeps registers low enough for two
+// blocks (6*BY waves) per CU; face fluxes meet in LDS.
+//
+// Per plane k (two LDS-only barriers):
+//   S. the first (BY+2)*66 threads stage their cell of plane k+1 (loaded one plane
+//      earlier) with its sound speed and total energy (hllc.hpp:64-65,82) into the
+//      other half of a double-buffered tile, then load their cell of plane k+2;
+//      Z threads load the conservatives of plane k;
+//   F. faces of plane k: x and y faces from tile k, z faces from tiles k and k+1;
+//   U. Z threads combine faces in the reference's order (x, y, z; godunov.hpp:48-82)
+//      and update their cell in place.
+// Every face is solved once with the operands of the reference's second solve of
+// it, so results are bitwise those of the reference order.
+//
+// Fused = true additionally converts the updated cell to primitives into prim_out
+// (cons_to_prim.hpp:28-31) and reduces the next dt over the new primitives
+// (time_step.hpp:28-38), publishing it through `sink`.
+//
+// Requires n0, n1, n2 >= 3 and n0*n1*sizeof(T) < 2^32 (32-bit in-plane offsets).
+template <class T, class I, int BY, bool Fused>
+__global__ __launch_bounds__(192 * BY) void k_godunov_faces(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    constexpr int BX = 64;
+    constexpr int LX = BX + 2;
+    constexpr int LY = BY + 2;
+    constexpr int NSTAGE = LX * LY;
+    constexpr int NF = 7; // d, p, u0, u1, u2, c, E
+    static_assert(NSTAGE <= 192 * BY, "one staged cell per thread");
+    __shared__ T tile[2][NF][LY][LX];
+    __shared__ T fx[5][BY][BX + 1]; // x face between columns c-1 and c of row r
+    __shared__ T fy[5][BY + 1][BX]; // y face between rows r-1 and r of column c
+
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    int const bx = t % tiles_x;
+    int const by = (t / tiles_x) % tiles_y;
+    int const bz = t / (tiles_x * tiles_y);
+    int const lane = threadIdx.x;
+    int const wave = threadIdx.y; // 0..3*BY-1
+    int const role = wave / BY;   // 0: X, 1: Y, 2: Z
+    int const row = wave % BY;
+    int const lid = lane + BX * wave;
+
+    I const i0 = 1 + static_cast<I>(bx) * BX;
+    I const j0 = 1 + static_cast<I>(by) * BY;
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    I const s2 = n0 * n1;
+
+    // Staged cell of this thread: tile (sr, sc), global column clamped into the array.
+    bool const stager = lid < NSTAGE;
+    int const sr = lid / LX;
+    int const sc = lid % LX;
+    uint32_t soff = 0;
+    if (stager) {
+        I si = i0 - 1 + sc;
+        I sj = j0 - 1 + sr;
+        si = (si < n0 - 1) ? si : n0 - 1;
+        sj = (sj < n1 - 1) ? sj : n1 - 1;
+        soff = static_cast<uint32_t>(si + n0 * sj) * static_cast<uint32_t>(sizeof(T));
+    }
+    // Cell updated by a Z thread.
+    I const ci = i0 + lane;
+    I const cj = j0 + row;
+    bool const valid = role == 2 && ci <= n0 - 2 && cj <= n1 - 2;
+    I const cic = (ci < n0 - 1) ? ci : n0 - 1;
+    I const cjc = (cj < n1 - 1) ? cj : n1 - 1;
+    uint32_t const coff = static_cast<uint32_t>(cic + n0 * cjc) * static_cast<uint32_t>(sizeof(T));
+
+    auto stage = [&](int b, Prim<T> const& q) {
+        tile[b][0][sr][sc] = q.d;
+        tile[b][1][sr][sc] = q.p;
+        tile[b][2][sr][sc] = q.u0;
+        tile[b][3][sr][sc] = q.u1;
+        tile[b][4][sr][sc] = q.u2;
+        tile[b][5][sr][sc] = sound_speed(gamma, q.d, q.p);
+        tile[b][6][sr][sc] = total_energy(q, gm1);
+    };
+    auto cell = [&](int b, int r, int c) {
+        return Prim<T> {tile[b][0][r][c], tile[b][1][r][c], tile[b][2][r][c], tile[b][3][r][c], tile[b][4][r][c]};
+    };
+    T const dtodv = resolve_dtodv(dta);
+    T local_min = Limits<T>::max();
+
+    // Prologue: plane kbeg staged, plane kbeg+1 in the stagers' registers, low z
+    // face of the first plane in the Z threads' registers.
+    Prim<T> q_stage {};
+    if (stager) {
+        stage(static_cast<int>(kbeg & 1), ld_prim(prim, s2 * kbeg, soff));
+        q_stage = ld_prim(prim, s2 * (kbeg + 1), soff);
+    }
+    Flux<T> fz_lo {};
+    if (role == 2) {
+        Prim<T> const qm = ld_prim(prim, s2 * (kbeg - 1), coff);
+        Prim<T> const q0 = ld_prim(prim, s2 * kbeg, coff);
+        fz_lo = hllc_flux<2>(qm, sound_speed(gamma, qm.d, qm.p), total_energy(qm, gm1), q0,
+                             sound_speed(gamma, q0.d, q0.p), total_energy(q0, gm1));
+    }
+
+    for (I k = kbeg; k < kend; ++k) {
+        int const b = static_cast<int>(k & 1);
+        int const bn = b ^ 1;
+        I const plane = s2 * k;
+        // S. stage plane k+1, prefetch plane k+2; Z threads fetch cons of plane k.
+        if (stager) {
+            stage(bn, q_stage);
+            if (k + 2 <= kend) {
+                q_stage = ld_prim(prim, plane + 2 * s2, soff);
+            }
+        }
+        Cons<T> u {};
+        if (role == 2) {
+            u = ld_cons(cons, plane, coff);
+        }
+        lds_barrier();
+
+        // F. faces of plane k.
+        Flux<T> fz_hi {};
+        if (role == 0) {
+            int const r = row + 1;
+            Flux<T> const f = hllc_flux<0>(cell(b, r, lane), tile[b][5][r][lane], tile[b][6][r][lane],
+                                           cell(b, r, lane + 1), tile[b][5][r][lane + 1], tile[b][6][r][lane + 1]);
+            fx[0][row][lane] = f.d;
+            fx[1][row][lane] = f.e;
+            fx[2][row][lane] = f.m0;
+            fx[3][row][lane] = f.m1;
+            fx[4][row][lane] = f.m2;
+            if (row == 0 && lane < BY) {
+                // High-side x faces of the tile: columns 64 | 65 of every row.
+                int const rr = lane + 1;
+                Flux<T> const g = hllc_flux<0>(cell(b, rr, BX), tile[b][5][rr][BX], tile[b][6][rr][BX],
+                                               cell(b, rr, BX + 1), tile[b][5][rr][BX + 1], tile[b][6][rr][BX + 1]);
+                fx[0][lane][BX] = g.d;
+                fx[1][lane][BX] = g.e;
+                fx[2][lane][BX] = g.m0;
+                fx[3][lane][BX] = g.m1;
+                fx[4][lane][BX] = g.m2;
+            }
+        } else if (role == 1) {
+            int const c = lane + 1;
+            Flux<T> const f = hllc_flux<1>(cell(b, row, c), tile[b][5][row][c], tile[b][6][row][c],
+                                           cell(b, row + 1, c), tile[b][5][row + 1][c], tile[b][6][row + 1][c]);
+            fy[0][row][lane] = f.d;
+            fy[1][row][lane] = f.e;
+            fy[2][row][lane] = f.m0;
+            fy[3][row][lane] = f.m1;
+            fy[4][row][lane] = f.m2;
+            if (row == BY - 1) {
+                // High-side y faces of the tile: rows BY | BY+1.
+                Flux<T> const g = hllc_flux<1>(cell(b, BY, c), tile[b][5][BY][c], tile[b][6][BY][c],
+                                               cell(b, BY + 1, c), tile[b][5][BY + 1][c], tile[b][6][BY + 1][c]);
+                fy[0][BY][lane] = g.d;
+                fy[1][BY][lane] = g.e;
+                fy[2][BY][lane] = g.m0;
+                fy[3][BY][lane] = g.m1;
+                fy[4][BY][lane] = g.m2;
+            }
+        } else {
+            int const r = row + 1;
+            int const c = lane + 1;
+            fz_hi = hllc_flux<2>(cell(b, r, c), tile[b][5][r][c], tile[b][6][r][c], cell(bn, r, c),
+                                 tile[bn][5][r][c], tile[bn][6][r][c]);
+        }
+        lds_barrier();
+
+        // U. combine and update (Z threads).
+        if (role == 2) {
+            Flux<T> const fxl {fx[0][row][lane], fx[1][row][lane], fx[2][row][lane], fx[3][row][lane],
+                               fx[4][row][lane]};
+            Flux<T> const fxh {fx[0][row][lane + 1], fx[1][row][lane + 1], fx[2][row][lane + 1],
+                               fx[3][row][lane + 1], fx[4][row][lane + 1]};
+            Flux<T> const fyl {fy[0][row][lane], fy[1][row][lane], fy[2][row][lane], fy[3][row][lane],
+                               fy[4][row][lane]};
+            Flux<T> const fyh {fy[0][row + 1][lane], fy[1][row + 1][lane], fy[2][row + 1][lane],
+                               fy[3][row + 1][lane], fy[4][row + 1][lane]};
+            Flux<T> acc {T(0), T(0), T(0), T(0), T(0)};
+            accumulate(acc, ds0, fxh, fxl);
+            accumulate(acc, ds1, fyh, fyl);
+            accumulate(acc, ds2, fz_hi, fz_lo);
+            apply_update(u, dtodv, acc);
+            if (valid) {
+                st_cons(cons, plane, coff, u);
+                if constexpr (Fused) {
+                    Prim<T> const q1 = cons_to_prim_pt(u, gm1);
+                    st_prim(prim_out, plane, coff, q1);
+                    T const c1 = sound_speed(gamma, q1.d, q1.p);
+                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                }
+            }
+            fz_lo = fz_hi;
+        }
+    }
+
+    if constexpr (Fused) {
+        T const bm = block_min<T, 3 * BY>(local_min);
         if (lid == 0) {
             grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
         }

@@ -288,7 +288,7 @@ def godunov(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, cons_arrays: Eu
     if not isinstance(riemann_solver, hllc) and riemann_solver is not hllc:
         raise TypeError("the Riemann solver must be hllc")
     suf = _validate(prim_arrays, cons_arrays)
-    if prim_arrays.d.data_ptr() == cons_arrays.d.data_ptr():
+    if prim_arrays.d.numel() > 0 and prim_arrays.d.data_ptr() == cons_arrays.d.data_ptr():
         raise ValueError("prim and cons must not alias")
     n0, n1, n2 = extents(prim_arrays)
     p, c = prim_ptrs(prim_arrays), cons_ptrs(cons_arrays)
@@ -340,20 +340,21 @@ def init_random(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, seed: int =
          ctypes.c_uint64(seed), d_range[0], d_range[1], p_range[0], p_range[1], u_sigma)
 
 
-def set_godunov_tile(by: int, kchunk: int) -> None:
-    """Launch shape of the tiled Godunov kernel (tile height, planes per block)."""
-    call("eb_set_godunov_tile", by, kchunk)
+def set_godunov_config(variant: int, by: int, kchunk: int) -> None:
+    """Godunov kernel variant (0 face-parallel, 1 cell-parallel), tile height and
+    planes marched per block. Results do not depend on it."""
+    call("eb_set_godunov_config", variant, by, kchunk)
 
 
-def get_godunov_tile():
-    by, kc = ctypes.c_int(), ctypes.c_int()
-    call("eb_get_godunov_tile", ctypes.byref(by), ctypes.byref(kc))
-    return by.value, kc.value
+def get_godunov_config():
+    v, by, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("eb_get_godunov_config", ctypes.byref(v), ctypes.byref(by), ctypes.byref(kc))
+    return v.value, by.value, kc.value
 
 
 __all__ = [
     "PerfectGas", "UniformMesh3d", "hllc", "EulerPrimArrays", "EulerConsArrays", "ExecSpace",
     "create_prim_arrays_1d", "create_cons_arrays_1d", "to_mdspan", "as_const", "size", "size_bytes",
     "cons_to_prim", "prim_to_cons", "time_step", "time_step_async", "godunov", "init_implode",
-    "init_diagonal", "init_random", "set_godunov_tile", "get_godunov_tile", "extents",
+    "init_diagonal", "init_random", "set_godunov_config", "get_godunov_config", "extents",
 ]

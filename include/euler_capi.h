@@ -54,10 +54,15 @@ const char* eb_status_string(int status);
 /* Message of the last failure on the calling thread ("" if none). */
 const char* eb_last_error(void);
 
-/* Godunov kernel launch shape: tile height `by` in {4, 8, 16} and planes per block
- * along k (`kchunk` >= 1). Process-wide; defaults 8 and 64. */
-int eb_set_godunov_tile(int by, int kchunk);
-int eb_get_godunov_tile(int* by, int* kchunk);
+/* Godunov kernel configuration, process-wide (used by eb_godunov*, eb_step_fused*):
+ *   variant 0: face-parallel waves, tile height `by` in {2, 4};
+ *   variant 1: cell-parallel waves, `by` in {4, 8, 16} (default);
+ *   variant 2: streamed (producer wave + global->LDS transfers), `by` in {4, 8};
+ *              needs 16-byte aligned primitive fields (else EB_ERR_UNSUPPORTED);
+ *   `kchunk` >= 1 planes marched per block along k. Defaults (1, 8, 128).
+ * All configurations give bitwise identical results. */
+int eb_set_godunov_config(int variant, int by, int kchunk);
+int eb_get_godunov_config(int* variant, int* by, int* kchunk);
 
 /* ---- cons_to_prim.hpp:7-33 --------------------------------------------- */
 int eb_cons_to_prim_f64(hipStream_t stream, const eb_cons_f64* cons, const eb_prim_f64* prim,

@@ -24,10 +24,11 @@ def _fields(a: np.ndarray):
 
 
 class Oracle:
-    def __init__(self, threads: int | None = None):
-        if not LIB_PATH.exists():
-            raise ImportError(f"{LIB_PATH} missing: run `make -C {LIB_PATH.parent}`")
-        self.lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    def __init__(self, threads: int | None = None, lib_path: Path | None = None):
+        path = Path(lib_path) if lib_path is not None else LIB_PATH
+        if not path.exists():
+            raise ImportError(f"{path} missing: run `make -C {LIB_PATH.parent}`")
+        self.lib = ctypes.CDLL(os.fspath(path))
         i64, vp = ctypes.c_int64, ctypes.c_void_p
         for suf, real in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             L = self.lib

@@ -49,3 +49,19 @@ def test_python_package_binds_all_symbols():
     bound = set(_lib.SIGNATURES) | {"eb_status_string", "eb_last_error"}
     assert set(declared_symbols()) <= bound, sorted(set(declared_symbols()) - bound)
     assert eb.godunov is not None
+
+
+def test_no_undefined_kernel_stubs():
+    """Every kernel launched from host code has its host stub in the library (a
+    device-only construct in a kernel body can make the host pass drop it)."""
+    import shutil
+    import subprocess
+
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", str(LIB)], capture_output=True, text=True, check=True).stdout
+    undefined = [l for l in out.splitlines() if " U " in l and "__device_stub__" in l]
+    assert not undefined, undefined[:5]

@@ -90,7 +90,8 @@ def test_time_step_uniform_closed_form(eb, ex):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("index_bits", [32, 64])
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("tile", [(8, 64), (4, 3), (16, 1)])
+@pytest.mark.parametrize("tile", [(0, 4, 64), (0, 2, 3), (1, 8, 64), (1, 4, 3), (1, 16, 1), (2, 4, 64), (2, 8, 3),
+                                  (2, 4, 1)])
 def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
     print(f"godunov dtype={dtype.__name__} bits={index_bits} shape={shape} tile={tile}", flush=True)
     gamma = 1.4
@@ -102,14 +103,14 @@ def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
     ref = cons.copy()
     orc.godunov(prim, ref, gamma, *dx, dt)
 
-    eb.set_godunov_tile(*tile)
+    eb.set_godunov_config(*tile)
     try:
         P = to_prim(eb, prim, index_bits)
         C = to_cons(eb, cons, index_bits)
         eb.godunov(ex, P, C, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx), eb.hllc(), float(dt))
         got = to_host(C)
     finally:
-        eb.set_godunov_tile(8, 64)
+        eb.set_godunov_config(1, 8, 128)
     assert_parity(got, ref, dtype, "godunov")
     # Ghost layer untouched.
     for sl in (np.s_[:, 0], np.s_[:, -1], np.s_[:, :, 0], np.s_[:, :, -1], np.s_[..., 0], np.s_[..., -1]):
@@ -174,10 +175,12 @@ def test_error_paths(eb, ex):
     prim = random_prim((6, 6, 6), seed=25)
     P, C = to_prim(eb, prim), to_cons(eb, prim)
     with pytest.raises(EulerError):
-        eb.set_godunov_tile(5, 8)
+        eb.set_godunov_config(0, 8, 8)
     with pytest.raises(EulerError):
-        eb.set_godunov_tile(8, 0)
-    assert eb.get_godunov_tile() == (8, 64)
+        eb.set_godunov_config(1, 5, 8)
+    with pytest.raises(EulerError):
+        eb.set_godunov_config(0, 4, 0)
+    assert eb.get_godunov_config() == (1, 8, 128)
     with pytest.raises(TypeError):
         eb.godunov(ex, P, to_cons(eb, prim.astype(np.float32)), eb.PerfectGas(1.4),
                    eb.UniformMesh3d(1, 1, 1), eb.hllc(), 0.1)
