@@ -112,6 +112,17 @@ def cpu_baseline(n: int, budget_s: float = 10.0):
     }
 
 
+def profiled_traffic(kernel: str, n: int):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated) for this kernel and workload, from the committed profile
+    summary; None when the live configuration was not profiled."""
+    path = ROOT / "profiles" / "godunov_f64_512_traffic.json"
+    if n != 512 or not path.exists():
+        return None
+    rec = json.loads(path.read_text())
+    return rec["traffic_bytes_per_launch"] if rec.get("kernel") == kernel else None
+
+
 def kernel_name(cfg) -> str:
     variant, by, _ = cfg
     kind = "k_godunov_faces" if variant == 0 else "k_godunov_tiled"
@@ -270,7 +281,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(kernel_name(eb.get_godunov_config()), n),
             "kernel": kernel_name(eb.get_godunov_config()),
             "bytes_per_launch": bytes_godunov,
             "avg_launch_ms": round(ms, 4),
