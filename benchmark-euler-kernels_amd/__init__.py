@@ -7,5 +7,6 @@ from . import _lib  # noqa: F401  (loads the HIP library; raises if it is missin
 from .operators import *  # noqa: F401,F403
 from .operators import __all__ as _ops_all
 from . import simulation  # noqa: F401
+from . import adjoint  # noqa: F401
 
 __all__ = list(_ops_all)

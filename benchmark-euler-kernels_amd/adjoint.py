@@ -1,0 +1,177 @@
+"""Adjoint of a Godunov rollout (reference step loop, euler_simulation.cpp:56-67)
+with the time step frozen per step, exposed as a torch.autograd.Function.
+
+Forward:  U_0 = p2c(P_0);  for n < N:  dt_n = time_step(P_n) (recorded, then held
+constant for differentiation);  U_{n+1} = godunov(P_n, U_n, cfl*dt_n);
+P_{n+1} = c2p(U_{n+1}).  The first step reads the given P_0, later steps read
+c2p(U_n), exactly as the reference loop does.
+
+Backward: conservative states are checkpointed every `checkpoint_every` steps;
+each segment is recomputed from its checkpoint (with the recorded dt, so it is
+bitwise the forward) and then reversed step by step through the hand-written HIP
+vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). Memory: one state
+per checkpoint plus one primitive state per step of a segment.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import operators as ops
+from ._lib import call
+
+
+def _clone_prim(a: ops.EulerPrimArrays) -> ops.EulerPrimArrays:
+    return ops.EulerPrimArrays(*(t.clone() for t in a.tensors()), index_bits=a.index_bits)
+
+
+def _clone_cons(a: ops.EulerConsArrays) -> ops.EulerConsArrays:
+    return ops.EulerConsArrays(*(t.clone() for t in a.tensors()), index_bits=a.index_bits)
+
+
+def _empty_cons_like(a: ops.Arrays) -> ops.EulerConsArrays:
+    return ops.EulerConsArrays(*(torch.empty_like(t) for t in a.tensors()), index_bits=a.index_bits)
+
+
+def _empty_prim_like(a: ops.Arrays) -> ops.EulerPrimArrays:
+    return ops.EulerPrimArrays(*(torch.empty_like(t) for t in a.tensors()), index_bits=a.index_bits)
+
+
+class Rollout:
+    """N-step rollout with checkpoints and its adjoint (dt frozen per step)."""
+
+    def __init__(self, exec_space: ops.ExecSpace, eos: ops.PerfectGas, mesh: ops.UniformMesh3d, cfl: float,
+                 nsteps: int, checkpoint_every: int = 20):
+        if nsteps < 1 or checkpoint_every < 1:
+            raise ValueError("nsteps and checkpoint_every must be >= 1")
+        self.ex = exec_space
+        self.eos = eos
+        self.mesh = mesh
+        self.cfl = cfl
+        self.nsteps = nsteps
+        self.k = checkpoint_every
+        self.dts: Optional[torch.Tensor] = None
+        self.checkpoints: dict = {}
+        self.prim0: Optional[ops.EulerPrimArrays] = None
+        self.prim_final: Optional[ops.EulerPrimArrays] = None
+
+    # ---------------------------------------------------------------- forward
+    def _step(self, prim: ops.EulerPrimArrays, cons: ops.EulerConsArrays, n: int, record: bool):
+        dt = self.dts[n:n + 1]
+        if record:
+            ops.time_step_async(self.ex, prim, self.eos, self.mesh, out=dt)
+        ops.godunov(self.ex, prim, cons, self.eos, self.mesh, ops.hllc(), dt, cfl=self.cfl)
+
+    def forward(self, prim0: ops.EulerPrimArrays, frozen_dts: Optional[torch.Tensor] = None) -> ops.EulerPrimArrays:
+        """Runs the rollout from P_0; returns P_N (new tensors). With frozen_dts
+        (raw time steps of an earlier rollout, one per step) the CFL reduction is
+        skipped and those time steps are used, which is the map the adjoint
+        differentiates."""
+        ops._validate(prim0)
+        self.prim0 = _clone_prim(prim0)
+        cons = _empty_cons_like(prim0)
+        ops.prim_to_cons(self.ex, self.prim0, cons, self.eos)
+        prim = _clone_prim(prim0)
+        if frozen_dts is not None:
+            if frozen_dts.numel() != self.nsteps or frozen_dts.dtype != prim0.d.dtype:
+                raise ValueError("frozen_dts must hold one raw time step per step in the arrays' real type")
+            self.dts = frozen_dts.to(prim0.d.device).clone()
+        else:
+            self.dts = torch.empty(self.nsteps, dtype=prim0.d.dtype, device=prim0.d.device)
+        self.checkpoints = {}
+        for n in range(self.nsteps):
+            if n % self.k == 0:
+                self.checkpoints[n] = _clone_cons(cons)
+            self._step(prim, cons, n, record=frozen_dts is None)
+            ops.cons_to_prim(self.ex, cons, prim, self.eos)
+        self.prim_final = prim
+        return _clone_prim(prim)
+
+    # --------------------------------------------------------------- backward
+    def _recompute(self, start: int, stop: int) -> List[ops.EulerPrimArrays]:
+        """P_n for n in [start, stop), recomputed from the checkpoint at `start`."""
+        cons = _clone_cons(self.checkpoints[start])
+        if start == 0:
+            prim = _clone_prim(self.prim0)
+        else:
+            prim = _empty_prim_like(cons)
+            ops.cons_to_prim(self.ex, cons, prim, self.eos)
+        prims = []
+        for n in range(start, stop):
+            prims.append(_clone_prim(prim))
+            if n + 1 < stop:
+                self._step(prim, cons, n, record=False)
+                ops.cons_to_prim(self.ex, cons, prim, self.eos)
+        return prims
+
+    def backward(self, gprim_final: Sequence[torch.Tensor]) -> ops.EulerPrimArrays:
+        """dL/dP_0 from dL/dP_N (five tensors in the primitive field order)."""
+        if self.prim_final is None:
+            raise RuntimeError("backward before forward")
+        suf = ops._suffix(self.prim0.d)
+        r = lambda x: ops._real(suf, x)  # noqa: E731
+        n0, n1, n2 = ops.extents(self.prim0)
+        bits = self.prim0.index_bits
+        m = self.mesh
+        geom = (n0, n1, n2, bits, r(self.eos.gamma), r(m.dx0), r(m.dx1), r(m.dx2))
+        h = self.ex.handle
+
+        gP = ops.EulerPrimArrays(*(g.contiguous() for g in gprim_final), index_bits=bits)
+        mu = _empty_cons_like(self.prim0)
+        mu_next = _empty_cons_like(self.prim0)
+        call(f"eb_cons_to_prim_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(self.prim_final)),
+             ctypes.byref(ops.prim_ptrs(gP)), ctypes.byref(ops.cons_ptrs(mu)), n0, n1, n2, bits,
+             r(self.eos.gamma))
+        gprim0 = _empty_prim_like(self.prim0)
+        starts = sorted(self.checkpoints)
+        for start in reversed(starts):
+            stop = min(start + self.k, self.nsteps)
+            prims = self._recompute(start, stop)
+            for n in range(stop - 1, start - 1, -1):
+                P = prims[n - start]
+                dt = self.dts[n:n + 1].data_ptr()
+                if n == 0:
+                    call(f"eb_godunov_vjp_first_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
+                         ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.prim_ptrs(gprim0)), *geom, dt,
+                         r(self.cfl))
+                else:
+                    call(f"eb_godunov_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
+                         ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.cons_ptrs(mu_next)), *geom, dt,
+                         r(self.cfl))
+                    mu, mu_next = mu_next, mu
+        return gprim0
+
+
+class _RolloutFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rollout: Rollout, index_bits: int, frozen_dts, d, p, u0, u1, u2):
+        P0 = ops.EulerPrimArrays(d.contiguous(), p.contiguous(), u0.contiguous(), u1.contiguous(), u2.contiguous(),
+                                 index_bits=index_bits)
+        PN = rollout.forward(P0, frozen_dts)
+        ctx.rollout = rollout
+        return PN.tensors()
+
+    @staticmethod
+    def backward(ctx, *grads):
+        rollout: Rollout = ctx.rollout
+        ref = rollout.prim0.d
+        g = [torch.zeros_like(ref) if x is None else x for x in grads]
+        g0 = rollout.backward(g)
+        return (None, None, None) + tuple(g0.tensors())
+
+
+def rollout(prim0: ops.EulerPrimArrays, eos: ops.PerfectGas, mesh: ops.UniformMesh3d, cfl: float, nsteps: int,
+            checkpoint_every: int = 20, exec_space: Optional[ops.ExecSpace] = None,
+            frozen_dts: Optional[torch.Tensor] = None, tape: Optional[list] = None):
+    """Differentiable N-step rollout: returns the five final primitive fields as
+    torch tensors whose gradient w.r.t. prim0's fields is the hand-written adjoint
+    (dt frozen per step). prim0's tensors may require grad. frozen_dts replays the
+    time steps of an earlier rollout; `tape`, if given, receives the Rollout
+    (its .dts holds the raw time steps used)."""
+    ex = exec_space if exec_space is not None else ops.ExecSpace(prim0.d.device)
+    ro = Rollout(ex, eos, mesh, cfl, nsteps, checkpoint_every)
+    if tape is not None:
+        tape.append(ro)
+    return _RolloutFunction.apply(ro, prim0.index_bits, frozen_dts, *prim0.tensors())

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "euler_capi.h"
+#include "adjoint_kernels.hpp"
 #include "kernels.hpp"
 
 struct eb_context
@@ -679,6 +680,162 @@ int abi_init_random(hipStream_t s, PS const* prim, int64_t n, int bits, uint64_t
     return check_launch("init_random");
 }
 
+// ---- adjoint --------------------------------------------------------------
+
+template <class T, class PS, class CS>
+int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu_out, PS const* gprim, int64_t n0,
+                    int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T const* dt_dev, T cfl)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    bool const first = gprim != nullptr;
+    if (prim_null(prim) || cons_null(mu_in) || dt_dev == nullptr || (first ? prim_null(gprim) : cons_null(mu_out))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (!first && mu_in->d == mu_out->d) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "mu_out must not alias mu_in");
+    }
+    int64_t const n = n0 * n1 * n2;
+    if (n == 0) {
+        return EB_OK;
+    }
+    T ds0;
+    T ds1;
+    T ds2;
+    T dv;
+    Mesh::faces(dx0, dx1, dx2, ds0, ds1, ds2, dv);
+    eb::DtArg<T> const dta {T(0), dt_dev, cfl, dv};
+    auto const p = prim_ptrs<T>(prim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    auto const mi = cons_ptrs<T>(mu_in);
+    eb::ConsPtrs<T const> const mic {mi.d, mi.e, mi.mx0, mi.mx1, mi.mx2};
+    eb::ConsPtrs<T> const mo = first ? eb::ConsPtrs<T> {} : cons_ptrs<T>(mu_out);
+    eb::PrimPtrs<T> const gp = first ? prim_ptrs<T>(gprim) : eb::PrimPtrs<T> {};
+    T const gm1 = gamma - 1;
+    dim3 const grid(pointwise_grid(n));
+#define EB_VJP(I, M)                                                                                              \
+    hipLaunchKernelGGL((eb::k_godunov_vjp<T, I, M>), grid, dim3(256), 0, s, pc, mic, mo, gp, static_cast<I>(n0), \
+                       static_cast<I>(n1), static_cast<I>(n2), gamma, gm1, ds0, ds1, ds2, dta)
+    if (bits == 32) {
+        if (first) {
+            EB_VJP(int32_t, 1);
+        } else {
+            EB_VJP(int32_t, 0);
+        }
+    } else {
+        if (first) {
+            EB_VJP(int64_t, 1);
+        } else {
+            EB_VJP(int64_t, 0);
+        }
+    }
+#undef EB_VJP
+    return check_launch("godunov_vjp");
+}
+
+template <class T, class PS, class CS>
+int abi_c2p_vjp(hipStream_t s, PS const* prim, PS const* gprim, CS const* gcons, int64_t n0, int64_t n1, int64_t n2,
+                int bits, T gamma)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    int64_t const n = n0 * n1 * n2;
+    if (n > 0 && (prim_null(prim) || prim_null(gprim) || cons_null(gcons))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    if (n == 0) {
+        return EB_OK;
+    }
+    auto const p = prim_ptrs<T>(prim);
+    auto const g = prim_ptrs<T>(gprim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    eb::PrimPtrs<T const> const gc {g.d, g.p, g.ux0, g.ux1, g.ux2};
+    if (bits == 32) {
+        hipLaunchKernelGGL((eb::k_c2p_vjp<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc, gc,
+                           cons_ptrs<T>(gcons), static_cast<int32_t>(n), static_cast<T>(gamma - 1));
+    } else {
+        hipLaunchKernelGGL((eb::k_c2p_vjp<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc, gc,
+                           cons_ptrs<T>(gcons), n, static_cast<T>(gamma - 1));
+    }
+    return check_launch("cons_to_prim_vjp");
+}
+
+template <class T, class PS, class CS>
+int abi_p2c_vjp(hipStream_t s, PS const* prim, CS const* gcons, PS const* gprim, int64_t n0, int64_t n1, int64_t n2,
+                int bits, T gamma)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    int64_t const n = n0 * n1 * n2;
+    if (n > 0 && (prim_null(prim) || prim_null(gprim) || cons_null(gcons))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    if (n == 0) {
+        return EB_OK;
+    }
+    auto const p = prim_ptrs<T>(prim);
+    auto const g = cons_ptrs<T>(gcons);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    eb::ConsPtrs<T const> const gc {g.d, g.e, g.mx0, g.mx1, g.mx2};
+    if (bits == 32) {
+        hipLaunchKernelGGL((eb::k_p2c_vjp<T, int32_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc, gc,
+                           prim_ptrs<T>(gprim), static_cast<int32_t>(n), static_cast<T>(gamma - 1));
+    } else {
+        hipLaunchKernelGGL((eb::k_p2c_vjp<T, int64_t>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc, gc,
+                           prim_ptrs<T>(gprim), n, static_cast<T>(gamma - 1));
+    }
+    return check_launch("prim_to_cons_vjp");
+}
+
+template <class T, class PS, class CS>
+int abi_hllc(hipStream_t s, int dir, PS const* qL, PS const* qR, CS const* g, CS const* flux, PS const* gL,
+             PS const* gR, int64_t nfaces, T gamma)
+{
+    if (dir < 0 || dir > 2 || nfaces < 0) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "dir must be 0, 1 or 2 and nfaces >= 0");
+    }
+    if (nfaces == 0) {
+        return EB_OK;
+    }
+    if (nfaces > (int64_t(1) << 40)) {
+        return fail(EB_ERR_INDEX_OVERFLOW, "too many faces");
+    }
+    bool const vjp = flux == nullptr;
+    if (prim_null(qL) || prim_null(qR) || (vjp ? (cons_null(g) || prim_null(gL) || prim_null(gR)) : cons_null(flux))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    auto cp = [](PS const* q) {
+        auto const a = prim_ptrs<T>(q);
+        return eb::PrimPtrs<T const> {a.d, a.p, a.ux0, a.ux1, a.ux2};
+    };
+    dim3 const grid(static_cast<unsigned int>((nfaces + 255) / 256));
+    if (vjp) {
+        auto const w = cons_ptrs<T>(g);
+        eb::ConsPtrs<T const> const wc {w.d, w.e, w.mx0, w.mx1, w.mx2};
+#define EB_HV(D)                                                                                                   \
+    hipLaunchKernelGGL((eb::k_hllc_vjp<T, D>), grid, dim3(256), 0, s, cp(qL), cp(qR), wc, prim_ptrs<T>(gL),      \
+                       prim_ptrs<T>(gR), nfaces, gamma)
+        if (dir == 0) {
+            EB_HV(0);
+        } else if (dir == 1) {
+            EB_HV(1);
+        } else {
+            EB_HV(2);
+        }
+#undef EB_HV
+        return check_launch("hllc_vjp");
+    }
+#define EB_HF(D)                                                                                                   \
+    hipLaunchKernelGGL((eb::k_hllc_flux<T, D>), grid, dim3(256), 0, s, cp(qL), cp(qR), cons_ptrs<T>(flux), nfaces, \
+                       gamma)
+    if (dir == 0) {
+        EB_HF(0);
+    } else if (dir == 1) {
+        EB_HF(1);
+    } else {
+        EB_HF(2);
+    }
+#undef EB_HF
+    return check_launch("hllc_flux");
+}
+
 #undef EB_CHECK
 
 } // namespace
@@ -754,6 +911,42 @@ extern "C" {
                              double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma)                     \
     {                                                                                                                \
         return abi_init_random<T>(s, p, n, bits, seed, d_lo, d_hi, p_lo, p_hi, u_sigma);                            \
+    }                                                                                                                \
+    int eb_godunov_vjp_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* mu_in,                    \
+                             const eb_cons_##SUF* mu_out, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma,     \
+                             T dx0, T dx1, T dx2, const T* dt_dev, T cfl)                                            \
+    {                                                                                                                \
+        return abi_godunov_vjp<T, eb_prim_##SUF, eb_cons_##SUF>(s, p, mu_in, mu_out, nullptr, n0, n1, n2, bits,     \
+                                                                gamma, dx0, dx1, dx2, dt_dev, cfl);                   \
+    }                                                                                                                \
+    int eb_godunov_vjp_first_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* mu_in,              \
+                                   const eb_prim_##SUF* gprim, int64_t n0, int64_t n1, int64_t n2, int bits,        \
+                                   T gamma, T dx0, T dx1, T dx2, const T* dt_dev, T cfl)                             \
+    {                                                                                                                \
+        return abi_godunov_vjp<T, eb_prim_##SUF, eb_cons_##SUF>(s, p, mu_in, nullptr, gprim, n0, n1, n2, bits,      \
+                                                                gamma, dx0, dx1, dx2, dt_dev, cfl);                   \
+    }                                                                                                                \
+    int eb_cons_to_prim_vjp_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_prim_##SUF* gp,                 \
+                                  const eb_cons_##SUF* gc, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma)  \
+    {                                                                                                                \
+        return abi_c2p_vjp<T>(s, p, gp, gc, n0, n1, n2, bits, gamma);                                               \
+    }                                                                                                                \
+    int eb_prim_to_cons_vjp_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* gc,                  \
+                                  const eb_prim_##SUF* gp, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma)  \
+    {                                                                                                                \
+        return abi_p2c_vjp<T>(s, p, gc, gp, n0, n1, n2, bits, gamma);                                               \
+    }                                                                                                                \
+    int eb_hllc_flux_##SUF(hipStream_t s, int dir, const eb_prim_##SUF* qL, const eb_prim_##SUF* qR,               \
+                           const eb_cons_##SUF* flux, int64_t nfaces, T gamma)                                       \
+    {                                                                                                                \
+        return abi_hllc<T, eb_prim_##SUF, eb_cons_##SUF>(s, dir, qL, qR, nullptr, flux, nullptr, nullptr, nfaces,   \
+                                                         gamma);                                                     \
+    }                                                                                                                \
+    int eb_hllc_vjp_##SUF(hipStream_t s, int dir, const eb_prim_##SUF* qL, const eb_prim_##SUF* qR,                \
+                          const eb_cons_##SUF* g, const eb_prim_##SUF* gL, const eb_prim_##SUF* gR, int64_t nfaces, \
+                          T gamma)                                                                                   \
+    {                                                                                                                \
+        return abi_hllc<T, eb_prim_##SUF, eb_cons_##SUF>(s, dir, qL, qR, g, nullptr, gL, gR, nfaces, gamma);        \
     }
 
 EB_ABI(f64, double)

@@ -163,6 +163,53 @@ int eb_init_random_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n, i
 int eb_init_random_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n, int index_bits, uint64_t seed,
                        double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma);
 
+
+/* ---- adjoint (no reference counterpart; the reference has no derivatives) ---
+ * Reverse mode of the step U_{n+1} = U_n - dtodv*R(P_n), P_{n+1} = c2p(U_{n+1}),
+ * with the time step frozen (dtodv = cfl*dt_dev[0]/dv as in eb_godunov_dev).
+ * mu_* are adjoints of the conservatives (cons layout), all cells.
+ *   eb_godunov_vjp:       mu_out = mu_in + c2p^T(-dtodv * R^T(P) mu_in|interior)
+ *   eb_godunov_vjp_first: gprim  = -dtodv * R^T(P_0) mu_in|interior + p2c^T mu_in
+ * (the first step reads the given P_0, later steps read c2p(U_n)). */
+int eb_godunov_vjp_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* mu_in,
+                       const eb_cons_f64* mu_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                       double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl);
+int eb_godunov_vjp_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* mu_in,
+                       const eb_cons_f32* mu_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                       float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl);
+int eb_godunov_vjp_first_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* mu_in,
+                             const eb_prim_f64* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                             double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl);
+int eb_godunov_vjp_first_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* mu_in,
+                             const eb_prim_f32* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                             float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl);
+/* gcons = (d prim / d cons)^T gprim at the states with primitives prim. */
+int eb_cons_to_prim_vjp_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_prim_f64* gprim,
+                            const eb_cons_f64* gcons, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                            double gamma);
+int eb_cons_to_prim_vjp_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_prim_f32* gprim,
+                            const eb_cons_f32* gcons, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                            float gamma);
+/* gprim = (d cons / d prim)^T gcons. */
+int eb_prim_to_cons_vjp_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* gcons,
+                            const eb_prim_f64* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                            double gamma);
+int eb_prim_to_cons_vjp_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* gcons,
+                            const eb_prim_f32* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
+                            float gamma);
+/* Batched HLLC flux (hllc.hpp:53-104) and its VJP over nfaces independent faces
+ * normal to dir (0, 1, 2): validation entry points. */
+int eb_hllc_flux_f64(hipStream_t stream, int dir, const eb_prim_f64* qL, const eb_prim_f64* qR,
+                     const eb_cons_f64* flux, int64_t nfaces, double gamma);
+int eb_hllc_flux_f32(hipStream_t stream, int dir, const eb_prim_f32* qL, const eb_prim_f32* qR,
+                     const eb_cons_f32* flux, int64_t nfaces, float gamma);
+int eb_hllc_vjp_f64(hipStream_t stream, int dir, const eb_prim_f64* qL, const eb_prim_f64* qR,
+                    const eb_cons_f64* g, const eb_prim_f64* gL, const eb_prim_f64* gR, int64_t nfaces,
+                    double gamma);
+int eb_hllc_vjp_f32(hipStream_t stream, int dir, const eb_prim_f32* qL, const eb_prim_f32* qR,
+                    const eb_cons_f32* g, const eb_prim_f32* gL, const eb_prim_f32* gR, int64_t nfaces,
+                    float gamma);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

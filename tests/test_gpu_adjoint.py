@@ -1,0 +1,206 @@
+"""Hand-written adjoint (HIP VJP kernels) against torch autograd of a plain FP64
+tensor implementation of the same scheme, and against central finite
+differences, with the time step frozen per step."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import torch_reference as tr
+
+pytestmark = pytest.mark.gpu
+
+G = 1.4
+
+
+def _faces(n, seed, umag=0.8):
+    g = torch.Generator().manual_seed(seed)
+    q = torch.empty(5, n, dtype=torch.float64)
+    q[0] = 0.3 + 2.0 * torch.rand(n, generator=g)
+    q[1] = 0.3 + 2.0 * torch.rand(n, generator=g)
+    q[2:] = umag * torch.randn(3, n, generator=g)
+    return q.cuda()
+
+
+def _prim(eb, q):
+    return eb.EulerPrimArrays(*[q[f].contiguous() for f in range(5)], index_bits=64)
+
+
+def _call_hllc_vjp(eb, d, qL, qR, w):
+    from benchmark_euler_kernels_amd._lib import call
+
+    n = qL.shape[1]
+    gL, gR = torch.empty_like(qL), torch.empty_like(qL)
+    PL, PR = _prim(eb, qL), _prim(eb, qR)
+    W = eb.EulerConsArrays(*[w[f].contiguous() for f in range(5)])
+    GL, GR = _prim(eb, gL), _prim(eb, gR)
+    ops = eb.operators
+    call("eb_hllc_vjp_f64", torch.cuda.current_stream().cuda_stream, d, ctypes.byref(ops.prim_ptrs(PL)),
+         ctypes.byref(ops.prim_ptrs(PR)), ctypes.byref(ops.cons_ptrs(W)), ctypes.byref(ops.prim_ptrs(GL)),
+         ctypes.byref(ops.prim_ptrs(GR)), n, ctypes.c_double(G))
+    torch.cuda.synchronize()
+    return torch.stack(GL.tensors()), torch.stack(GR.tensors())
+
+
+def _switch_margin(qL, qR, d):
+    cL = torch.sqrt(G * qL[1] / qL[0])
+    cR = torch.sqrt(G * qR[1] / qR[0])
+    unL, unR = qL[2 + d], qR[2 + d]
+    SL = torch.minimum(unL, unR) - torch.maximum(cL, cR)
+    SR = torch.maximum(unL, unR) + torch.maximum(cL, cR)
+    F = tr.hllc(list(qL), list(qR), d, G)  # noqa: F841 (forward for parity only)
+    rcL, rcR = qL[0] * (SL - unL), qR[0] * (SR - unR)
+    us = (qR[1] - qL[1] + rcL * unL - rcR * unR) / (rcL - rcR)
+    return torch.stack([(cL - cR).abs(), (unL - unR).abs(), us.abs(), (SL * SR).abs()]).min(dim=0).values
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hllc_vjp_matches_autograd(eb, ex, d):
+    qL, qR = _faces(4000, 100 + d), _faces(4000, 200 + d)
+    w = _faces(4000, 300 + d)
+    gL, gR = _call_hllc_vjp(eb, d, qL, qR, w)
+    aL, aR = qL.clone().requires_grad_(True), qR.clone().requires_grad_(True)
+    F = tr.hllc(list(aL), list(aR), d, G)
+    s = sum((f * w[i]).sum() for i, f in enumerate(F))
+    rL, rR = torch.autograd.grad(s, (aL, aR))
+    scale = max(rL.abs().max().item(), rR.abs().max().item())
+    assert torch.allclose(gL, rL, rtol=1e-11, atol=1e-12 * scale)
+    assert torch.allclose(gR, rR, rtol=1e-11, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hllc_vjp_central_differences(eb, ex, d):
+    """Directional derivatives of the HIP flux by central FD (eps = 1e-6) agree
+    with the VJP away from branch switches; every branch is exercised."""
+    from benchmark_euler_kernels_amd._lib import call
+
+    n = 3000
+    qL, qR = _faces(n, 400 + d, umag=1.5), _faces(n, 500 + d, umag=1.5)
+    w = _faces(n, 600 + d)
+    gL, gR = _call_hllc_vjp(eb, d, qL, qR, w)
+    ops = eb.operators
+
+    def flux(a, b):
+        out = torch.empty_like(a)
+        Fo = eb.EulerConsArrays(*[out[f] for f in range(5)])
+        call("eb_hllc_flux_f64", torch.cuda.current_stream().cuda_stream, d, ctypes.byref(ops.prim_ptrs(_prim(eb, a))),
+             ctypes.byref(ops.prim_ptrs(_prim(eb, b))), ctypes.byref(ops.cons_ptrs(Fo)), n, ctypes.c_double(G))
+        torch.cuda.synchronize()
+        return out
+
+    g = torch.Generator().manual_seed(700 + d)
+    vL = torch.randn(5, n, generator=g, dtype=torch.float64).cuda()
+    vR = torch.randn(5, n, generator=g, dtype=torch.float64).cuda()
+    eps = 1e-6
+    fd = ((flux(qL + eps * vL, qR + eps * vR) - flux(qL - eps * vL, qR - eps * vR)) * w).sum(0) / (2 * eps)
+    an = (gL * vL).sum(0) + (gR * vR).sum(0)
+    keep = _switch_margin(qL, qR, d) > 1e-3
+    assert keep.float().mean() > 0.9
+    err = (fd - an).abs()[keep] / (an.abs()[keep] + 1e-3 * an.abs().max())
+    assert err.max().item() < 1e-5
+    sup = (qL[2 + d] - torch.sqrt(G * qL[1] / qL[0]) > 0) | (qR[2 + d] + torch.sqrt(G * qR[1] / qR[0]) < 0)
+    assert sup.any() and (~sup).any()
+
+
+def _smooth_state(shape, seed):
+    n0, n1, n2 = shape
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    x, y, z = (x + 0.5) / n0, (y + 0.5) / n1, (z + 0.5) / n2
+    a = rng.uniform(0.1, 0.3, 8)
+    P = np.empty((5, n2, n1, n0))
+    P[0] = 1.0 + a[0] * np.sin(2 * np.pi * (x + 2 * y)) + a[1] * np.cos(2 * np.pi * z)
+    P[1] = 1.0 + a[2] * np.cos(2 * np.pi * (y - z)) + a[3] * np.sin(2 * np.pi * x)
+    P[2] = a[4] * np.sin(2 * np.pi * (y + z))
+    P[3] = a[5] * np.cos(2 * np.pi * (x - z))
+    P[4] = a[6] * np.sin(2 * np.pi * (x + y + z))
+    return P
+
+
+def _octant_mask(shape, dx):
+    n0, n1, n2 = shape
+    c = [((np.arange(n) - 1) + 0.5) * h for n, h in zip((n0, n1, n2), dx)]
+    m = (c[2][:, None, None] < 0.25) & (c[1][None, :, None] < 0.25) & (c[0][None, None, :] < 0.25)
+    m &= np.ones((n2, n1, n0), bool)
+    inner = np.zeros_like(m)
+    inner[1:-1, 1:-1, 1:-1] = True
+    return torch.from_numpy(m & inner).cuda()
+
+
+def _hip_grad(eb, ex, P0, dx, nsteps, k, cfl=0.5, tape=None):
+    ts = [torch.from_numpy(P0[f].copy()).cuda().requires_grad_(True) for f in range(5)]
+    prim = eb.EulerPrimArrays(*ts, index_bits=32)
+    out = eb.adjoint.rollout(prim, eb.PerfectGas(G), eb.UniformMesh3d(*dx), cfl, nsteps, k, exec_space=ex,
+                             tape=tape)
+    return ts, out
+
+
+@pytest.mark.parametrize("k", [1, 4, 100])
+def test_rollout_gradient_matches_torch_reference(eb, ex, k):
+    shape = (18, 14, 12)
+    dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
+    nsteps, cfl = 6, 0.5
+    P0 = _smooth_state(shape, 1)
+    mask = _octant_mask(shape, dx)
+    ts, out = _hip_grad(eb, ex, P0, dx, nsteps, k, cfl)
+    loss = (out[0] * out[0] * mask).sum()
+    loss.backward()
+    ours = torch.stack([t.grad for t in ts])
+
+    ref_in = [torch.from_numpy(P0[f].copy()).cuda().requires_grad_(True) for f in range(5)]
+    PN = tr.rollout(ref_in, G, dx, cfl, nsteps)
+    ref_loss = (PN[0] * PN[0] * mask).sum()
+    ref_loss.backward()
+    ref = torch.stack([t.grad for t in ref_in])
+    assert abs(loss.item() - ref_loss.item()) <= 1e-13 * abs(ref_loss.item())
+    err = (ours - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-11, err
+    assert ref.abs().sum() > 0
+
+
+def test_rollout_checkpoint_interval_is_bitwise_invariant(eb, ex):
+    shape = (18, 14, 12)
+    dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
+    P0 = _smooth_state(shape, 2)
+    mask = _octant_mask(shape, dx)
+    grads = []
+    for k in (1, 3, 7, 50):
+        ts, out = _hip_grad(eb, ex, P0, dx, 7, k)
+        (out[0] * out[0] * mask).sum().backward()
+        grads.append(torch.stack([t.grad for t in ts]))
+    for g in grads[1:]:
+        assert torch.equal(g, grads[0])
+
+
+def test_rollout_central_differences(eb, ex):
+    """<dL/dP0, v> against (L(P0 + eps v) - L(P0 - eps v)) / (2 eps), eps = 1e-6,
+    on random smooth directions (densities and pressures perturbed relatively)."""
+    shape = (18, 18, 18)
+    dx = (1.0 / 16,) * 3
+    nsteps = 10
+    P0 = _smooth_state(shape, 3)
+    mask = _octant_mask(shape, dx)
+    tape = []
+    ts, out = _hip_grad(eb, ex, P0, dx, nsteps, 4, tape=tape)
+    (out[0] * out[0] * mask).sum().backward()
+    g = torch.stack([t.grad for t in ts]).cpu().numpy()
+    dts = tape[0].dts.clone()
+
+    def loss(P):
+        # dt frozen: replay the unperturbed rollout's time steps.
+        with torch.no_grad():
+            prim = eb.EulerPrimArrays(*[torch.from_numpy(P[f].copy()).cuda() for f in range(5)])
+            o = eb.adjoint.rollout(prim, eb.PerfectGas(G), eb.UniformMesh3d(*dx), 0.5, nsteps, 4, exec_space=ex,
+                                   frozen_dts=dts)
+            return float((o[0] * o[0] * mask).sum().item())
+
+    rng = np.random.default_rng(5678)
+    for _ in range(6):
+        v = rng.normal(0.0, 1.0, P0.shape) * np.abs(P0).mean(axis=(1, 2, 3))[:, None, None, None]
+        eps = 1e-6
+        fd = (loss(P0 + eps * v) - loss(P0 - eps * v)) / (2 * eps)
+        an = float((g * v).sum())
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-9, (fd, an)
