@@ -119,3 +119,30 @@ def rollout(P0, gamma, dx, cfl, nsteps):
         U = godunov(P, U, gamma, dx, dt)
         P = cons_to_prim(U, gamma)
     return P
+
+
+def switch_margins(P, gamma, box=None):
+    """Smallest distance to each HLLC branch switch over the faces of state P
+    between cells of `box` ((lo, hi) per tensor dimension, default all):
+    |cL - cR| (max of sound speeds), |unL - unR| (min/max of normal speeds),
+    |u*| (upwind side) and |S_L * S_R| (supersonic test), as floats."""
+    if box is not None:
+        P = [q[tuple(slice(lo, hi) for lo, hi in box)] for q in P]
+    out = {"c": float("inf"), "un": float("inf"), "us": float("inf"), "sup": float("inf")}
+    for d in range(3):
+        ax = AXIS[d]
+        n = P[0].shape[ax]
+        qL = [_slice(q.detach(), ax, 0, n - 1) for q in P]
+        qR = [_slice(q.detach(), ax, 1, n) for q in P]
+        cL = torch.sqrt(gamma * qL[1] / qL[0])
+        cR = torch.sqrt(gamma * qR[1] / qR[0])
+        unL, unR = qL[2 + d], qR[2 + d]
+        cmax = kmax(cL, cR)
+        SL = kmin(unL, unR) - cmax
+        SR = kmax(unL, unR) + cmax
+        rcL, rcR = qL[0] * (SL - unL), qR[0] * (SR - unR)
+        us = (qR[1] - qL[1] + rcL * unL - rcR * unR) / (rcL - rcR)
+        for key, v in (("c", (cL - cR).abs()), ("un", (unL - unR).abs()), ("us", us.abs()),
+                       ("sup", (SL * SR).abs())):
+            out[key] = min(out[key], float(v.min()))
+    return out
