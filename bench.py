@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--n", type=int, default=512, help="interior cells per axis")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline godunov only (for profiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (HBM counter calibration)")
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
@@ -112,6 +113,87 @@ def cpu_baseline(n: int, budget_s: float = 10.0):
     }
 
 
+def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 20, ndirs: int = 64):
+    """Forward + hand-written adjoint of a 200-step FP64 rollout of the diagonal
+    implosion at 128^3 (density perturbed by N(0,0.01) * bump, seed 1234), loss =
+    sum rho^2 over the octant x, y, z < 0.25, checkpoints every 20 steps, against
+    the plain forward step loop (fused step, hipGraph) over the same steps; then
+    central FD (eps = 1e-6, dt frozen) on `ndirs` density directions (seeds 5678+j)."""
+    import statistics
+
+    G = 1.4
+    eos = eb.PerfectGas(G)
+    mesh = eb.UniformMesh3d(1.0 / n, 1.0 / n, 1.0 / n)
+    P0 = eb.workloads.adjoint_state(ex, n)
+    c = (torch.arange(n + 2, dtype=torch.float64, device=P0.d.device) - 0.5) / n
+    inner = (c > 0) & (c < 0.25)
+    mask = inner[:, None, None] & inner[None, :, None] & inner[None, None, :]
+
+    def clone_prim():
+        return eb.EulerPrimArrays(*(t.clone() for t in P0.tensors()), index_bits=P0.index_bits)
+
+    # plain forward: fused step loop, graph-replayed pairs
+    prim = clone_prim()
+    cons = eb.EulerConsArrays(*(torch.empty_like(t) for t in P0.tensors()), index_bits=P0.index_bits)
+    eb.prim_to_cons(ex, prim, cons, eos)
+    sim = eb.simulation.Simulation(ex, prim, cons, eos, mesh, 0.5)
+    sim.run(4, use_graph=True)
+    ms_plain = timer.run(lambda: sim.run(nsteps, use_graph=True), 1)
+    del sim, prim, cons
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, r
+
+    def fwd_only():
+        with torch.no_grad():
+            return eb.adjoint.rollout(clone_prim(), eos, mesh, 0.5, nsteps, k, exec_space=ex)
+
+    def fwd_bwd():
+        ts = [t.clone().requires_grad_(True) for t in P0.tensors()]
+        tape = []
+        out = eb.adjoint.rollout(eb.EulerPrimArrays(*ts, index_bits=P0.index_bits), eos, mesh, 0.5, nsteps, k,
+                                 exec_space=ex, tape=tape)
+        (out[0] * out[0] * mask).sum().backward()
+        return [t.grad for t in ts], tape[0].dts.clone()
+
+    fwd_only()
+    fwd_bwd()
+    ms_fwd, _ = wall(fwd_only)
+    ms_total, (grads, dts) = wall(fwd_bwd)
+
+    def loss(rho):
+        with torch.no_grad():
+            P = eb.EulerPrimArrays(rho, *(t.clone() for t in P0.tensors()[1:]), index_bits=P0.index_bits)
+            o = eb.adjoint.rollout(P, eos, mesh, 0.5, nsteps, nsteps, exec_space=ex, frozen_dts=dts)
+            return float((o[0] * o[0] * mask).sum().item())
+
+    eps = 1e-6
+    errs = []
+    t0 = time.perf_counter()
+    for j in range(ndirs):
+        v = eb.workloads.density_direction(n, j, P0.d.device)
+        fd = (loss(P0.d + eps * v) - loss(P0.d - eps * v)) / (2 * eps)
+        an = float((grads[0] * v).sum())
+        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+    fd_s = time.perf_counter() - t0
+    return {
+        "workload": f"{nsteps}-step FP64 rollout, diagonal implosion {n}^3, rho += N(0,0.01)*bump (seed 1234), "
+                    f"loss sum rho^2 over octant x,y,z<0.25, checkpoints every {k}",
+        "plain_forward_ms": round(ms_plain, 2),
+        "rollout_forward_ms": round(ms_fwd, 2),
+        "forward_plus_adjoint_ms": round(ms_total, 2),
+        "ratio_vs_plain_forward": round(ms_total / ms_plain, 3),
+        "ratio_vs_rollout_forward": round(ms_total / ms_fwd, 3),
+        "fd_check": {"directions": ndirs, "eps": eps, "field": "rho (N(0,1)*bump, seeds 5678+j)",
+                     "max_rel_err": float(max(errs)), "median_rel_err": float(statistics.median(errs)),
+                     "n_below_1e-6": sum(e < 1e-6 for e in errs), "seconds": round(fd_s, 1)},
+    }
+
+
 def profiled_traffic(kernel: str, n: int):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     copy-calibrated) for this kernel and workload, from the committed profile
@@ -125,7 +207,7 @@ def profiled_traffic(kernel: str, n: int):
 
 def kernel_name(cfg) -> str:
     variant, by, _ = cfg
-    kind = "k_godunov_faces" if variant == 0 else "k_godunov_tiled"
+    kind = {0: "k_godunov_faces", 1: "k_godunov_tiled", 2: "k_godunov_stream"}[variant]
     return f"eb::{kind}<double, long, {by}, false>"
 
 
@@ -238,6 +320,12 @@ def main():
             "speedup_vs_f64": round(ms / ms32, 3),
         }
         del prim32, cons32
+        torch.cuda.empty_cache()
+
+    if not args.quick and not args.no_adjoint:
+        log("adjoint: 128^3 implosion, 200 steps, forward + adjoint and FD check")
+        extra["adjoint"] = adjoint_leg(torch, eb, ex, timer)
+        log(f"adjoint: {extra['adjoint']}")
         torch.cuda.empty_cache()
 
     if args.copy_calibration:

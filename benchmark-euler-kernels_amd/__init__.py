@@ -8,5 +8,6 @@ from .operators import *  # noqa: F401,F403
 from .operators import __all__ as _ops_all
 from . import simulation  # noqa: F401
 from . import adjoint  # noqa: F401
+from . import workloads  # noqa: F401
 
 __all__ = list(_ops_all)

@@ -40,7 +40,11 @@ def _empty_prim_like(a: ops.Arrays) -> ops.EulerPrimArrays:
 
 
 class Rollout:
-    """N-step rollout with checkpoints and its adjoint (dt frozen per step)."""
+    """N-step rollout with checkpoints and its adjoint (dt frozen per step).
+
+    Steps run on the fused kernel (godunov + cons_to_prim + next time step, the
+    launch sequence of simulation.Simulation), so the forward costs what the plain
+    step loop costs plus one copy of the conservative state per checkpoint."""
 
     def __init__(self, exec_space: ops.ExecSpace, eos: ops.PerfectGas, mesh: ops.UniformMesh3d, cfl: float,
                  nsteps: int, checkpoint_every: int = 20):
@@ -56,66 +60,91 @@ class Rollout:
         self.checkpoints: dict = {}
         self.prim0: Optional[ops.EulerPrimArrays] = None
         self.prim_final: Optional[ops.EulerPrimArrays] = None
+        self._slots: Optional[torch.Tensor] = None  # [0]: discarded dt, [1]: ghost-cell dt minimum
+        self._segment: List[ops.EulerPrimArrays] = []
+
+    # ---------------------------------------------------------------- launches
+    def _geom(self):
+        n0, n1, n2 = ops.extents(self.prim0)
+        m = self.mesh
+        r = lambda x: ops._real(self.suf, x)  # noqa: E731
+        return n0, n1, n2, self.prim0.index_bits, r(self.eos.gamma), r(m.dx0), r(m.dx1), r(m.dx2)
+
+    def _ghost_refresh(self, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays):
+        call(f"eb_ghost_refresh_{self.suf}", self.ex.context, self.ex.handle, ctypes.byref(ops.cons_ptrs(cons)),
+             ctypes.byref(ops.prim_ptrs(dst)), *self._geom(), self._slots[1:].data_ptr())
+
+    def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
+               dt_out: int):
+        call(f"eb_step_fused_{self.suf}", self.ex.context, self.ex.handle, ctypes.byref(ops.prim_ptrs(src)),
+             ctypes.byref(ops.cons_ptrs(cons)), ctypes.byref(ops.prim_ptrs(dst)), *self._geom(),
+             ops._real(self.suf, self.cfl), dt_in, self._slots[1:].data_ptr(), dt_out, None)
 
     # ---------------------------------------------------------------- forward
-    def _step(self, prim: ops.EulerPrimArrays, cons: ops.EulerConsArrays, n: int, record: bool):
-        dt = self.dts[n:n + 1]
-        if record:
-            ops.time_step_async(self.ex, prim, self.eos, self.mesh, out=dt)
-        ops.godunov(self.ex, prim, cons, self.eos, self.mesh, ops.hllc(), dt, cfl=self.cfl)
-
     def forward(self, prim0: ops.EulerPrimArrays, frozen_dts: Optional[torch.Tensor] = None) -> ops.EulerPrimArrays:
         """Runs the rollout from P_0; returns P_N (new tensors). With frozen_dts
-        (raw time steps of an earlier rollout, one per step) the CFL reduction is
-        skipped and those time steps are used, which is the map the adjoint
-        differentiates."""
-        ops._validate(prim0)
+        (raw time steps of an earlier rollout, one per step) those time steps are
+        used instead of the CFL ones, which is the map the adjoint differentiates."""
+        self.suf = ops._validate(prim0)
+        if min(ops.extents(prim0)) < 3:
+            raise ValueError("a rollout needs at least one interior cell")
+        dtype, dev = prim0.d.dtype, prim0.d.device
         self.prim0 = _clone_prim(prim0)
+        self._slots = torch.zeros(2, dtype=dtype, device=dev)
+        self._segment = []
         cons = _empty_cons_like(prim0)
         ops.prim_to_cons(self.ex, self.prim0, cons, self.eos)
-        prim = _clone_prim(prim0)
         if frozen_dts is not None:
-            if frozen_dts.numel() != self.nsteps or frozen_dts.dtype != prim0.d.dtype:
+            if frozen_dts.numel() != self.nsteps or frozen_dts.dtype != dtype:
                 raise ValueError("frozen_dts must hold one raw time step per step in the arrays' real type")
-            self.dts = frozen_dts.to(prim0.d.device).clone()
+            dt = frozen_dts.to(dev).clone()
         else:
-            self.dts = torch.empty(self.nsteps, dtype=prim0.d.dtype, device=prim0.d.device)
+            dt = torch.empty(self.nsteps + 1, dtype=dtype, device=dev)
+            ops.time_step_async(self.ex, self.prim0, self.eos, self.mesh, out=dt[0:1])
+        self.dts = dt[:self.nsteps]
+        buf = [_clone_prim(prim0), _empty_prim_like(prim0)]
+        self._ghost_refresh(cons, buf[1])
         self.checkpoints = {}
         for n in range(self.nsteps):
             if n % self.k == 0:
                 self.checkpoints[n] = _clone_cons(cons)
-            self._step(prim, cons, n, record=frozen_dts is None)
-            ops.cons_to_prim(self.ex, cons, prim, self.eos)
-        self.prim_final = prim
-        return _clone_prim(prim)
+            out = self._slots[0:].data_ptr() if frozen_dts is not None else dt[n + 1:].data_ptr()
+            self._fused(buf[n % 2], cons, buf[1 - n % 2], dt[n:].data_ptr(), out)
+            if n == 0:
+                # later steps read ghost primitives round-tripped through cons_to_prim
+                self._ghost_refresh(cons, buf[0])
+        self.prim_final = buf[self.nsteps % 2]
+        return _clone_prim(self.prim_final)
 
     # --------------------------------------------------------------- backward
     def _recompute(self, start: int, stop: int) -> List[ops.EulerPrimArrays]:
-        """P_n for n in [start, stop), recomputed from the checkpoint at `start`."""
+        """P_n for n in [start, stop), recomputed from the checkpoint at `start`
+        with the recorded time steps (bitwise the forward's states)."""
         cons = _clone_cons(self.checkpoints[start])
+        need = stop - start
+        while len(self._segment) < need:
+            b = _empty_prim_like(cons)
+            self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
+            self._segment.append(b)
         if start == 0:
-            prim = _clone_prim(self.prim0)
+            prims = [self.prim0]
         else:
-            prim = _empty_prim_like(cons)
-            ops.cons_to_prim(self.ex, cons, prim, self.eos)
-        prims = []
-        for n in range(start, stop):
-            prims.append(_clone_prim(prim))
-            if n + 1 < stop:
-                self._step(prim, cons, n, record=False)
-                ops.cons_to_prim(self.ex, cons, prim, self.eos)
+            ops.cons_to_prim(self.ex, cons, self._segment[0], self.eos)
+            prims = [self._segment[0]]
+        for n in range(start, stop - 1):
+            nxt = self._segment[n - start + 1]
+            self._fused(prims[-1], cons, nxt, self.dts[n:].data_ptr(), self._slots[0:].data_ptr())
+            prims.append(nxt)
         return prims
 
     def backward(self, gprim_final: Sequence[torch.Tensor]) -> ops.EulerPrimArrays:
         """dL/dP_0 from dL/dP_N (five tensors in the primitive field order)."""
         if self.prim_final is None:
             raise RuntimeError("backward before forward")
-        suf = ops._suffix(self.prim0.d)
+        suf = self.suf
         r = lambda x: ops._real(suf, x)  # noqa: E731
-        n0, n1, n2 = ops.extents(self.prim0)
-        bits = self.prim0.index_bits
-        m = self.mesh
-        geom = (n0, n1, n2, bits, r(self.eos.gamma), r(m.dx0), r(m.dx1), r(m.dx2))
+        geom = self._geom()
+        n0, n1, n2, bits = geom[:4]
         h = self.ex.handle
 
         gP = ops.EulerPrimArrays(*(g.contiguous() for g in gprim_final), index_bits=bits)

@@ -28,7 +28,7 @@ thread_local std::string g_last_error;
 // fastest measured at 512^3 on MI355X), 2 = streamed with a producer wave.
 std::atomic<int> g_variant {1};
 std::atomic<int> g_tile_by {8};
-std::atomic<int> g_tile_kchunk {128};
+std::atomic<int> g_tile_kchunk {0}; // 0: sized per launch by auto_kchunk
 
 bool config_supported(int variant, int by)
 {
@@ -235,6 +235,19 @@ void launch_tiled(
     }
 }
 
+// Planes marched per block when the configured kchunk is 0. A block re-reads one
+// primitive plane below its chunk, so long chunks save traffic; the chunk is
+// shortened until the launch has about 512 blocks (two per CU) so that small
+// grids still fill the GPU: 128 planes at 512^3 (2048 blocks), 8 at 128^3.
+int auto_kchunk(int64_t n0, int64_t n1, int64_t n2, int by)
+{
+    int64_t const nz = n2 - 2;
+    int64_t const columns = ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by);
+    int64_t kc = nz * columns / 512;
+    kc = kc < 8 ? 8 : (kc > 128 ? 128 : kc);
+    return static_cast<int>(kc < nz ? kc : (nz > 0 ? nz : 1));
+}
+
 template <class T, class I, bool Fused>
 int dispatch_tiled(
         hipStream_t s,
@@ -256,7 +269,8 @@ int dispatch_tiled(
 {
     int const variant = g_variant.load();
     int const by = g_tile_by.load();
-    int const kchunk = g_tile_kchunk.load();
+    int const kc_set = g_tile_kchunk.load();
+    int const kchunk = kc_set > 0 ? kc_set : auto_kchunk(n0, n1, n2, by);
     // The block count must fit a 32-bit grid and a plane must fit 32-bit offsets.
     int64_t const tiles = ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by) * ((n2 - 2 + kchunk - 1) / kchunk);
     if (tiles > std::numeric_limits<int32_t>::max()) {
@@ -461,8 +475,8 @@ const char* eb_last_error(void)
 
 int eb_set_godunov_config(int variant, int by, int kchunk)
 {
-    if (kchunk < 1) {
-        return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 1");
+    if (kchunk < 0) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 0 (0: automatic)");
     }
     if (!config_supported(variant, by)) {
         return fail(EB_ERR_UNSUPPORTED, "unsupported (variant, tile height): (0, 2|4) or (1, 4|8|16)");

@@ -341,8 +341,9 @@ def init_random(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, seed: int =
 
 
 def set_godunov_config(variant: int, by: int, kchunk: int) -> None:
-    """Godunov kernel variant (0 face-parallel, 1 cell-parallel), tile height and
-    planes marched per block. Results do not depend on it."""
+    """Godunov kernel variant (0 face-parallel, 1 cell-parallel, 2 streamed), tile
+    height and planes marched per block (0: sized per launch from the grid).
+    Results do not depend on it."""
     call("eb_set_godunov_config", variant, by, kchunk)
 
 

@@ -59,7 +59,9 @@ const char* eb_last_error(void);
  *   variant 1: cell-parallel waves, `by` in {4, 8, 16} (default);
  *   variant 2: streamed (producer wave + global->LDS transfers), `by` in {4, 8};
  *              needs 16-byte aligned primitive fields (else EB_ERR_UNSUPPORTED);
- *   `kchunk` >= 1 planes marched per block along k. Defaults (1, 8, 128).
+ *   `kchunk` >= 1 planes marched per block along k, or 0: chosen per launch from
+ *   the grid (long chunks, shortened until the launch has ~512 blocks).
+ *   Defaults (1, 8, 0).
  * All configurations give bitwise identical results. */
 int eb_set_godunov_config(int variant, int by, int kchunk);
 int eb_get_godunov_config(int* variant, int* by, int* kchunk);

@@ -110,7 +110,7 @@ def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
         eb.godunov(ex, P, C, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx), eb.hllc(), float(dt))
         got = to_host(C)
     finally:
-        eb.set_godunov_config(1, 8, 128)
+        eb.set_godunov_config(1, 8, 0)
     assert_parity(got, ref, dtype, "godunov")
     # Ghost layer untouched.
     for sl in (np.s_[:, 0], np.s_[:, -1], np.s_[:, :, 0], np.s_[:, :, -1], np.s_[..., 0], np.s_[..., -1]):
@@ -179,8 +179,8 @@ def test_error_paths(eb, ex):
     with pytest.raises(EulerError):
         eb.set_godunov_config(1, 5, 8)
     with pytest.raises(EulerError):
-        eb.set_godunov_config(0, 4, 0)
-    assert eb.get_godunov_config() == (1, 8, 128)
+        eb.set_godunov_config(0, 4, -1)
+    assert eb.get_godunov_config() == (1, 8, 0)
     with pytest.raises(TypeError):
         eb.godunov(ex, P, to_cons(eb, prim.astype(np.float32)), eb.PerfectGas(1.4),
                    eb.UniformMesh3d(1, 1, 1), eb.hllc(), 0.1)

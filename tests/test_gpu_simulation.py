@@ -72,7 +72,7 @@ def test_fused_loop_int64_and_tiles(eb, ex, orc):
             sim.run(4)
             sim.finish()
         finally:
-            eb.set_godunov_config(1, 8, 128)
+            eb.set_godunov_config(1, 8, 0)
         assert np.array_equal(to_host(C), c_ref), tile
         assert np.array_equal(to_host(P), p_ref), tile
 
