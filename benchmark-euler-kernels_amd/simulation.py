@@ -101,6 +101,9 @@ class Simulation:
         while self.steps_done < 2 and nsteps > 0:
             self.step()
             nsteps -= 1
+        if use_graph and nsteps >= 2 and self.steps_done % 2 == 1:
+            self.step()  # the captured pair starts from buffer 0
+            nsteps -= 1
         if use_graph and nsteps >= 2:
             g = self._pair_graph()
             while nsteps >= 2:

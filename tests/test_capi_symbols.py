@@ -65,3 +65,21 @@ def test_no_undefined_kernel_stubs():
     out = subprocess.run([nm, "-D", str(LIB)], capture_output=True, text=True, check=True).stdout
     undefined = [l for l in out.splitlines() if " U " in l and "__device_stub__" in l]
     assert not undefined, undefined[:5]
+
+
+def test_cpp_headers_compile_standalone(tmp_path):
+    """Each public C++ header compiles on its own with the host compiler."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    inc = ROOT / "include"
+    for h in sorted((inc / "euler").glob("*.hpp")):
+        src = tmp_path / f"inc_{h.stem}.cpp"
+        src.write_text(f"#include <euler/{h.name}>\nint main() {{ return 0; }}\n")
+        r = subprocess.run([gxx, "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                            "-D__HIP_PLATFORM_AMD__", f"-I{inc}", "-I/opt/rocm/include", str(src)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (h.name, r.stderr[-2000:])
