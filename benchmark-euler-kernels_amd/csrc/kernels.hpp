@@ -845,7 +845,7 @@ __global__ __launch_bounds__(64 * (BY + 1)) void k_godunov_stream(
     }
 }
 
-// Face-parallel Godunov update (2.5-D march along k), the default kernel.
+// Face-parallel Godunov update (2.5-D march along k), configuration variant 0.
 //
 // A block owns a 64 x BY tile of interior cells in (i, j) and marches along k over
 // `kchunk` planes. It has 3*BY waves in three roles, one wave per tile row each:
