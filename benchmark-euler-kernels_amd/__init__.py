@@ -9,5 +9,6 @@ from .operators import __all__ as _ops_all
 from . import simulation  # noqa: F401
 from . import adjoint  # noqa: F401
 from . import workloads  # noqa: F401
+from . import batch  # noqa: F401
 
 __all__ = list(_ops_all)

@@ -44,6 +44,20 @@ class ConsPtrs(ctypes.Structure):
     _fields_ = _fields(["d", "e", "mx0", "mx1", "mx2"])
 
 
+def _instance_struct(real):
+    class BatchInstance(ctypes.Structure):
+        """eb_batch_instance_f64 / _f32."""
+
+        _fields_ = [("prim_a", PrimPtrs), ("prim_b", PrimPtrs), ("cons", ConsPtrs), ("n0", ctypes.c_int64),
+                    ("n1", ctypes.c_int64), ("n2", ctypes.c_int64), ("gamma", real), ("dx0", real), ("dx1", real),
+                    ("dx2", real), ("cfl", real)]
+
+    return BatchInstance
+
+
+BatchInstanceF64 = _instance_struct(ctypes.c_double)
+BatchInstanceF32 = _instance_struct(ctypes.c_float)
+
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -59,6 +73,18 @@ def _signatures():
         "eb_context_destroy": [_vp],
         "eb_set_godunov_config": [_int, _int, _int],
         "eb_get_godunov_config": [_P(_int), _P(_int), _P(_int)],
+        "eb_batch_create_f64": [_P(_vp), _P(BatchInstanceF64), _int, _int],
+        "eb_batch_create_f32": [_P(_vp), _P(BatchInstanceF32), _int, _int],
+        "eb_batch_destroy": [_vp],
+        "eb_batch_set_tiles": [_vp, _int, _int],
+        "eb_batch_get_tiles": [_vp, _P(_int), _P(_int), _P(_int), _P(_int)],
+        "eb_batch_device_state": [_vp, _P(_vp), _P(_vp)],
+        "eb_batch_copy_state": [_vp, _vp, _vp, _vp],
+        "eb_batch_start": [_vp, _vp],
+        "eb_batch_refresh_ghosts": [_vp, _vp, _int],
+        "eb_batch_step": [_vp, _vp, _int],
+        "eb_batch_step_per_instance": [_vp, _vp, _int],
+        "eb_batch_autotune": [_vp, _vp, _int, _int, _P(_f32)],
     }
     for suf, real in (("f64", _f64), ("f32", _f32)):
         ext = [_i64, _i64, _i64, _int]

@@ -969,3 +969,5 @@ EB_ABI(f32, float)
 #undef EB_ABI
 
 } // extern "C"
+
+#include "batch_abi.hpp"

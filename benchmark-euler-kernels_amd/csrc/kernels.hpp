@@ -374,28 +374,34 @@ __device__ __forceinline__ void lds_barrier()
 // (time_step.hpp:28-38), publishing it through `sink`.
 //
 // Requires n0, n1, n2 >= 3 and n0*n1*sizeof(T) < 2^32 (32-bit in-plane offsets).
+//
+// The tile body is a device function so that the single-grid kernel below and the
+// batched ensemble kernel (one launch over the tiles of many grids) share it. It
+// returns the thread's contribution to the next dt (largest finite value unless
+// Fused). `apply` = false leaves the conservatives' values unchanged while doing
+// the same loads, solves and stores (used to time launch shapes on live state).
 template <class T, class I, int BY, bool Fused>
-__global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
-        PrimPtrs<T const> prim,
-        ConsPtrs<T> cons,
-        PrimPtrs<T> prim_out,
+__device__ __forceinline__ T godunov_tile(
+        PrimPtrs<T const> const& prim,
+        ConsPtrs<T> const& cons,
+        PrimPtrs<T> const& prim_out,
         I n0,
         I n1,
         I n2,
         int kchunk,
-        int tiles_x,
-        int tiles_y,
-        int tiles_z,
+        int bx,
+        int by,
+        int bz,
         T gamma,
         T gm1,
         T ds0,
         T ds1,
         T ds2,
-        DtArg<T> dta,
+        T dtodv,
         T invdx0,
         T invdx1,
         T invdx2,
-        DtSink<T> sink)
+        bool apply)
 {
     constexpr int BX = 64;
     constexpr int LX = BX + 2;
@@ -406,11 +412,6 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     __shared__ T sfy[5][BY + 1][BX]; // y faces: row r sits between tile rows r-1 and r
     __shared__ T sfx[5][BY];         // x faces on the tile's high-i edge
 
-    int const nblk = tiles_x * tiles_y * tiles_z;
-    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
-    int const bx = t % tiles_x;
-    int const by = (t / tiles_x) % tiles_y;
-    int const bz = t / (tiles_x * tiles_y);
     int const tx = threadIdx.x;
     int const ty = threadIdx.y;
     int const lid = tx + BX * ty;
@@ -455,7 +456,6 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
     }
 
-    T const dtodv = resolve_dtodv(dta);
     T local_min = Limits<T>::max();
 
     auto stage = [&](int r, int c, Prim<T> const& q, T cs, T e) {
@@ -548,7 +548,9 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         accumulate(acc, ds0, fx_hi, fx_lo);
         accumulate(acc, ds1, fy_hi, fy_lo);
         accumulate(acc, ds2, fz_hi, fz_lo);
-        apply_update(u, dtodv, acc);
+        if (apply) {
+            apply_update(u, dtodv, acc);
+        }
         if (valid) {
             st_cons(cons, plane, own, u);
             if constexpr (Fused) {
@@ -565,10 +567,43 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
         c_cur = c_next;
         e_cur = e_next;
     }
+    return local_min;
+}
 
+template <class T, class I, int BY, bool Fused>
+__global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    int const bx = t % tiles_x;
+    int const by = (t / tiles_x) % tiles_y;
+    int const bz = t / (tiles_x * tiles_y);
+    T const local_min = godunov_tile<T, I, BY, Fused>(prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz, gamma,
+                                                       gm1, ds0, ds1, ds2, resolve_dtodv(dta), invdx0, invdx1,
+                                                       invdx2, true);
     if constexpr (Fused) {
         T const bm = block_min<T, BY>(local_min);
-        if (lid == 0) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
             grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
         }
     }

@@ -212,6 +212,57 @@ int eb_hllc_vjp_f32(hipStream_t stream, int dir, const eb_prim_f32* qL, const eb
                     const eb_cons_f32* g, const eb_prim_f32* gL, const eb_prim_f32* gR, int64_t nfaces,
                     float gamma);
 
+
+/* ---- batched ensemble (no reference counterpart) ---------------------------
+ * Many independent problems, each with its own extents, gamma, cell widths and
+ * CFL factor, advanced together: one persistent launch per step over a queue of
+ * tiles drawn from every instance. Each instance keeps its own raw time step on
+ * the device (time_step.hpp:8-41 semantics) and its FP64 simulated time; results
+ * are bitwise those of the single-grid fused step (eb_step_fused_*). The caller
+ * owns every array: prim_a holds the initial primitives, prim_b is a second
+ * primitive buffer of the same extents, cons holds prim_to_cons(prim_a). Step
+ * parity p reads buffer p (a = 0, b = 1) and writes the other.
+ * Protocol: eb_batch_start; eb_batch_step(parity 0); eb_batch_refresh_ghosts(0);
+ * then eb_batch_step with parity 1, 0, 1, ... (steps are graph-capturable). */
+typedef struct eb_batch eb_batch;
+typedef struct eb_batch_instance_f64 {
+    eb_prim_f64 prim_a; eb_prim_f64 prim_b; eb_cons_f64 cons;
+    int64_t n0, n1, n2;
+    double gamma, dx0, dx1, dx2, cfl;
+} eb_batch_instance_f64;
+typedef struct eb_batch_instance_f32 {
+    eb_prim_f32 prim_a; eb_prim_f32 prim_b; eb_cons_f32 cons;
+    int64_t n0, n1, n2;
+    float gamma, dx0, dx1, dx2, cfl;
+} eb_batch_instance_f32;
+int eb_batch_create_f64(eb_batch** batch, const eb_batch_instance_f64* instances, int ninstances, int index_bits);
+int eb_batch_create_f32(eb_batch** batch, const eb_batch_instance_f32* instances, int ninstances, int index_bits);
+int eb_batch_destroy(eb_batch* batch);
+/* Tile height by in {4, 8, 16}; kchunk planes per tile (0: whole columns).
+ * Default (8, 32). eb_batch_autotune picks both by timing. */
+int eb_batch_set_tiles(eb_batch* batch, int by, int kchunk);
+int eb_batch_get_tiles(const eb_batch* batch, int* by, int* kchunk, int* ntiles, int* grid);
+/* Device arrays: dt = 3 reals per instance ([0], [1] ping-pong raw dt, [2] the
+ * ghost cells' dt minimum), time = 1 double per instance. */
+int eb_batch_device_state(const eb_batch* batch, void** dt, double** time);
+/* Copies those arrays (3*ninstances reals, ninstances doubles) to caller memory,
+ * host or device, enqueued on stream; either destination may be null. */
+int eb_batch_copy_state(const eb_batch* batch, hipStream_t stream, void* dt_dst, double* time_dst);
+/* First raw dt of every instance from prim_a (ghosts included), ghost cells of
+ * prim_b from the ghost conservatives. */
+int eb_batch_start(eb_batch* batch, hipStream_t stream);
+/* Ghost cells of buffer (0: prim_a, 1: prim_b) from the ghost conservatives. */
+int eb_batch_refresh_ghosts(eb_batch* batch, hipStream_t stream, int buffer);
+/* One step of every instance in one launch. */
+int eb_batch_step(eb_batch* batch, hipStream_t stream, int parity);
+/* The same step as one eb_step_fused_* launch per instance, with the
+ * process-wide single-grid configuration (for comparison). */
+int eb_batch_step_per_instance(eb_batch* batch, hipStream_t stream, int parity);
+/* Times every (tile height, kchunk) candidate on the live ensemble with a zero
+ * time step (the state is left bitwise unchanged; buffer 1-parity is scratch),
+ * keeps the fastest and returns its time per launch. Blocking. */
+int eb_batch_autotune(eb_batch* batch, hipStream_t stream, int parity, int reps, float* best_ms);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
