@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline godunov only (for profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (HBM counter calibration)")
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
@@ -194,6 +195,54 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
     }
 
 
+def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_steps: int = 50):
+    """The measured ensemble: `count` diagonal implosions (instance i drawn from
+    seed 1234+i: interior extents U{32..160} per axis, gamma ~ U[1.2,1.67],
+    interface offset ~ U[0.3,0.6], inner rho, p ~ U[0.05,0.3]), FP32, CFL 0.5,
+    per-instance dt on the device. One persistent launch per step (autotuned
+    tiles, hipGraph-replayed) against one fused launch per instance and step."""
+    prims, conss, eoss, meshes, cfls, params = eb.batch.random_ensemble(ex, count, seed=1234, dtype=torch.float32,
+                                                                        index_bits=32, cfl=0.5)
+    cells = sum(p["n"][0] * p["n"][1] * p["n"][2] for p in params)
+    ens = eb.batch.Ensemble(ex, prims, conss, eoss, meshes, cfls)
+    ens.start()
+    ens.step()
+    t0 = time.perf_counter()
+    tune_ms = ens.autotune(reps=3)
+    tune_s = time.perf_counter() - t0
+    tiles = ens.tiles()
+    ens.run(3, use_graph=True)  # capture
+    ms_graph = timer.run(lambda: ens.run(nsteps - 4, use_graph=True), 1) / (nsteps - 4)
+    ms_eager = timer.run(lambda: ens.run(base_steps), 1) / base_steps
+    ens.run(2, use_graph=True, per_instance=True)  # capture the per-instance pair
+    ms_pi_graph = timer.run(lambda: ens.run(base_steps, use_graph=True, per_instance=True), 1) / base_steps
+    ms_pi_eager = timer.run(lambda: ens.run(base_steps, per_instance=True), 1) / base_steps
+    _, t = ens.state()
+    steps_done = ens.steps_done
+    ens.close()
+    rate = lambda ms: round(cells / (ms * 1e-3) / 1e6, 1)  # noqa: E731
+    return {
+        "workload": f"{count} diagonal implosions, interior extents U{{32..160}} per axis (seed 1234+i), "
+                    f"gamma U[1.2,1.67], offset U[0.3,0.6], inner rho,p U[0.05,0.3], FP32, CFL 0.5",
+        "interior_cells": cells,
+        "autotuned_tiles": tiles,
+        "autotune_ms_per_launch": round(tune_ms, 4),
+        "autotune_seconds": round(tune_s, 2),
+        "batched_graph_ms_per_step": round(ms_graph, 4),
+        "batched_graph_Mcell_updates_per_s": rate(ms_graph),
+        "batched_eager_ms_per_step": round(ms_eager, 4),
+        "per_instance_eager_ms_per_step": round(ms_pi_eager, 4),
+        "per_instance_eager_Mcell_updates_per_s": rate(ms_pi_eager),
+        "per_instance_graph_ms_per_step": round(ms_pi_graph, 4),
+        "per_instance_graph_Mcell_updates_per_s": rate(ms_pi_graph),
+        "speedup_vs_per_instance_eager": round(ms_pi_eager / ms_graph, 3),
+        "speedup_vs_per_instance_graph": round(ms_pi_graph / ms_graph, 3),
+        "steps_done": steps_done,
+        "simulated_time_min_max": [float(t.min()), float(t.max())],
+        "timed_steps": {"batched_graph": nsteps - 4, "others": base_steps},
+    }
+
+
 def profiled_traffic(kernel: str, n: int):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     copy-calibrated) for this kernel and workload, from the committed profile
@@ -320,6 +369,12 @@ def main():
             "speedup_vs_f64": round(ms / ms32, 3),
         }
         del prim32, cons32
+        torch.cuda.empty_cache()
+
+    if not args.quick and not args.no_batch:
+        log("batched ensemble: 256 FP32 implosions of uneven shapes")
+        extra["batch"] = batch_leg(torch, eb, ex, timer)
+        log(f"batch: {extra['batch']}")
         torch.cuda.empty_cache()
 
     if not args.quick and not args.no_adjoint:
