@@ -108,6 +108,8 @@ def _signatures():
         )
         sig[f"eb_cons_to_prim_vjp_{suf}"] = [_vp, _P(PrimPtrs), _P(PrimPtrs), _P(ConsPtrs)] + ext + [real]
         sig[f"eb_prim_to_cons_vjp_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real]
+        sig[f"eb_conserved_totals_{suf}"] = [_vp, _vp, _P(ConsPtrs)] + ext + [_vp]
+        sig[f"eb_boundary_flux_{suf}"] = [_vp, _vp, _P(PrimPtrs)] + ext + [real] * 4 + [_vp, real, _vp]
         sig[f"eb_hllc_flux_{suf}"] = [_vp, _int, _P(PrimPtrs), _P(PrimPtrs), _P(ConsPtrs), _i64, real]
         sig[f"eb_hllc_vjp_{suf}"] = (
             [_vp, _int, _P(PrimPtrs), _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs), _P(PrimPtrs), _i64, real]

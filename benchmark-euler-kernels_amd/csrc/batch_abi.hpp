@@ -280,7 +280,7 @@ int batch_create(eb_batch** out, inst_c<T> const* inst, int n, int index_bits)
 
 inline eb_context instance_context(eb_batch* b, int i)
 {
-    return eb_context {b->device, b->scratch + i, nullptr};
+    return eb_context {b->device, b->scratch + i, nullptr, nullptr};
 }
 
 template <class T, class I>

@@ -10,6 +10,7 @@
 
 #include "euler_capi.h"
 #include "adjoint_kernels.hpp"
+#include "diagnostics.hpp"
 #include "kernels.hpp"
 
 struct eb_context
@@ -17,6 +18,7 @@ struct eb_context
     int device;
     eb::MinScratch* scratch; // one slot: reductions through one context are serialised
     double* dt_slot;         // device landing slot for the blocking time step
+    double* partials;        // per-block (sum, compensation) pairs of the FP64 diagnostics
 };
 
 namespace {
@@ -50,6 +52,9 @@ int check_launch(char const* what)
     }
     return EB_OK;
 }
+
+// Blocks of the FP64 diagnostic reductions (partials held by the context).
+constexpr int kDiagBlocks = 2048;
 
 // Largest total cell count accepted with 32-bit indices: the reference's int_cast
 // rule plus headroom for the grid-stride arithmetic (idx + 4*stride, stride <= 2^20).
@@ -409,6 +414,48 @@ int run_step_fused(
                                       ds1, ds2, dta, T(1) / dx0, T(1) / dx1, T(1) / dx2, sink);
 }
 
+template <class T, class I, class CS>
+int run_conserved_totals(eb_context* ctx, hipStream_t s, CS const* cons, int64_t n0, int64_t n1, int64_t n2,
+                         double* totals)
+{
+    int blocks = 0;
+    if (n0 >= 3 && n1 >= 3 && n2 >= 3) {
+        int64_t const rows = (n1 - 2) * (n2 - 2);
+        blocks = static_cast<int>(rows < kDiagBlocks ? rows : kDiagBlocks);
+        auto const c = cons_ptrs<T>(cons);
+        eb::ConsPtrs<T const> const cc {c.d, c.e, c.mx0, c.mx1, c.mx2};
+        hipLaunchKernelGGL((eb::k_conserved_partials<T, I>), dim3(blocks), dim3(eb::kDiagThreads), 0, s, cc,
+                           static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), ctx->partials);
+    }
+    hipLaunchKernelGGL(eb::k_compensated_final, dim3(1), dim3(64), 0, s, ctx->partials, blocks, totals, 0);
+    return check_launch("conserved_totals");
+}
+
+template <class T, class I, class PS>
+int run_boundary_flux(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, T gamma,
+                      T dx0, T dx1, T dx2, T const* dt_dev, T cfl, double* acc)
+{
+    if (n0 < 3 || n1 < 3 || n2 < 3) {
+        return EB_OK; // no interior: nothing flows
+    }
+    T ds0;
+    T ds1;
+    T ds2;
+    T dv;
+    Mesh::faces(dx0, dx1, dx2, ds0, ds1, ds2, dv);
+    int64_t const faces = 2 * ((n1 - 2) * (n2 - 2) + (n0 - 2) * (n2 - 2) + (n0 - 2) * (n1 - 2));
+    int64_t const want = (faces + eb::kDiagThreads - 1) / eb::kDiagThreads;
+    int const blocks = static_cast<int>(want < kDiagBlocks ? want : kDiagBlocks);
+    auto const p = prim_ptrs<T>(prim);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    eb::DtArg<T> const dta {T(0), dt_dev, cfl, dv};
+    hipLaunchKernelGGL((eb::k_boundary_partials<T, I>), dim3(blocks), dim3(eb::kDiagThreads), 0, s, pc,
+                       static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma,
+                       static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, ctx->partials);
+    hipLaunchKernelGGL(eb::k_compensated_final, dim3(1), dim3(64), 0, s, ctx->partials, blocks, acc, 1);
+    return check_launch("boundary_flux");
+}
+
 } // namespace
 
 // ======================================================================= ABI
@@ -427,20 +474,24 @@ int eb_context_create(eb_context** ctx)
     }
     eb::MinScratch* scratch = nullptr;
     double* dt_slot = nullptr;
+    double* partials = nullptr;
     if (hipMalloc(&scratch, sizeof(eb::MinScratch)) != hipSuccess) {
         return fail(EB_ERR_OUT_OF_MEMORY, "hipMalloc of reduction scratch failed");
     }
-    if (hipMalloc(&dt_slot, sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&dt_slot, sizeof(double)) != hipSuccess
+        || hipMalloc(&partials, sizeof(double) * 10 * kDiagBlocks) != hipSuccess) {
         (void)hipFree(scratch);
-        return fail(EB_ERR_OUT_OF_MEMORY, "hipMalloc of dt slot failed");
+        (void)hipFree(dt_slot);
+        return fail(EB_ERR_OUT_OF_MEMORY, "hipMalloc of context scratch failed");
     }
     hipLaunchKernelGGL(eb::min_scratch_init, dim3(1), dim3(64), 0, nullptr, scratch, 1);
     if (hipDeviceSynchronize() != hipSuccess) {
         (void)hipFree(scratch);
         (void)hipFree(dt_slot);
+        (void)hipFree(partials);
         return fail(EB_ERR_HIP, "reduction scratch initialisation failed");
     }
-    *ctx = new eb_context {dev, scratch, dt_slot};
+    *ctx = new eb_context {dev, scratch, dt_slot, partials};
     return EB_OK;
 }
 
@@ -449,7 +500,8 @@ int eb_context_destroy(eb_context* ctx)
     if (ctx == nullptr) {
         return EB_OK;
     }
-    bool const ok = hipFree(ctx->scratch) == hipSuccess && hipFree(ctx->dt_slot) == hipSuccess;
+    bool const ok = hipFree(ctx->scratch) == hipSuccess && hipFree(ctx->dt_slot) == hipSuccess
+                    && hipFree(ctx->partials) == hipSuccess;
     int const rc = ok ? EB_OK : fail(EB_ERR_HIP, "hipFree failed");
     delete ctx;
     return rc;
@@ -850,6 +902,32 @@ int abi_hllc(hipStream_t s, int dir, PS const* qL, PS const* qR, CS const* g, CS
     return check_launch("hllc_flux");
 }
 
+template <class T, class CS>
+int abi_conserved_totals(eb_context* ctx, hipStream_t s, CS const* cons, int64_t n0, int64_t n1, int64_t n2,
+                         int bits, double* totals)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (ctx == nullptr || totals == nullptr || (n0 >= 3 && n1 >= 3 && n2 >= 3 && cons_null(cons))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    return bits == 32 ? run_conserved_totals<T, int32_t>(ctx, s, cons, n0, n1, n2, totals)
+                      : run_conserved_totals<T, int64_t>(ctx, s, cons, n0, n1, n2, totals);
+}
+
+template <class T, class PS>
+int abi_boundary_flux(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, int bits,
+                      T gamma, T dx0, T dx1, T dx2, T const* dt_dev, T cfl, double* acc)
+{
+    EB_CHECK(check_extents(n0, n1, n2, bits));
+    if (ctx == nullptr || acc == nullptr || dt_dev == nullptr
+        || (n0 >= 3 && n1 >= 3 && n2 >= 3 && prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    return bits == 32 ? run_boundary_flux<T, int32_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev, cfl, acc)
+                      : run_boundary_flux<T, int64_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev, cfl,
+                                                      acc);
+}
+
 #undef EB_CHECK
 
 } // namespace
@@ -961,6 +1039,17 @@ extern "C" {
                           T gamma)                                                                                   \
     {                                                                                                                \
         return abi_hllc<T, eb_prim_##SUF, eb_cons_##SUF>(s, dir, qL, qR, g, nullptr, gL, gR, nfaces, gamma);        \
+    }                                                                                                                \
+    int eb_conserved_totals_##SUF(eb_context* ctx, hipStream_t s, const eb_cons_##SUF* c, int64_t n0, int64_t n1,   \
+                                  int64_t n2, int bits, double* totals)                                              \
+    {                                                                                                                \
+        return abi_conserved_totals<T>(ctx, s, c, n0, n1, n2, bits, totals);                                        \
+    }                                                                                                                \
+    int eb_boundary_flux_##SUF(eb_context* ctx, hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1,      \
+                               int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, const T* dt_dev, T cfl,          \
+                               double* acc)                                                                          \
+    {                                                                                                                \
+        return abi_boundary_flux<T>(ctx, s, p, n0, n1, n2, bits, gamma, dx0, dx1, dx2, dt_dev, cfl, acc);           \
     }
 
 EB_ABI(f64, double)

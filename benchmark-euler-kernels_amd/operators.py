@@ -340,6 +340,34 @@ def init_random(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, seed: int =
          ctypes.c_uint64(seed), d_range[0], d_range[1], p_range[0], p_range[1], u_sigma)
 
 
+def conserved_totals(exec_space: ExecSpace, cons_arrays: EulerConsArrays,
+                     out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Interior totals of (d, e, mx0, mx1, mx2) in FP64 with Neumaier-compensated,
+    fixed-order sums (device tensor of 5 doubles; no reference counterpart)."""
+    suf = _validate(cons_arrays)
+    n0, n1, n2 = extents(cons_arrays)
+    if out is None:
+        out = torch.empty(5, dtype=torch.float64, device=cons_arrays.d.device)
+    call(f"eb_conserved_totals_{suf}", exec_space.context, exec_space.handle, ctypes.byref(cons_ptrs(cons_arrays)),
+         n0, n1, n2, cons_arrays.index_bits, out.data_ptr())
+    return out
+
+
+def boundary_flux(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, eos: PerfectGas, mesh: UniformMesh3d,
+                  dt_dev: torch.Tensor, cfl: float, acc: torch.Tensor) -> None:
+    """Adds to acc (10 doubles: (sum, compensation) per conservative field) the
+    amount by which a godunov step with dt = cfl * dt_dev[0] lowers the interior
+    totals: dt/dv * sum_d ds_d (boundary flux out - boundary flux in)."""
+    suf = _validate(prim_arrays)
+    n0, n1, n2 = extents(prim_arrays)
+    if acc.dtype != torch.float64 or acc.numel() < 10 or not acc.is_cuda:
+        raise ValueError("acc must be a device tensor of 10 float64")
+    r = lambda x: _real(suf, x)  # noqa: E731
+    call(f"eb_boundary_flux_{suf}", exec_space.context, exec_space.handle, ctypes.byref(prim_ptrs(prim_arrays)),
+         n0, n1, n2, prim_arrays.index_bits, r(eos.gamma), r(mesh.dx0), r(mesh.dx1), r(mesh.dx2),
+         dt_dev.data_ptr(), r(cfl), acc.data_ptr())
+
+
 def set_godunov_config(variant: int, by: int, kchunk: int) -> None:
     """Godunov kernel variant (0 face-parallel, 1 cell-parallel, 2 streamed), tile
     height and planes marched per block (0: sized per launch from the grid).
@@ -358,4 +386,5 @@ __all__ = [
     "create_prim_arrays_1d", "create_cons_arrays_1d", "to_mdspan", "as_const", "size", "size_bytes",
     "cons_to_prim", "prim_to_cons", "time_step", "time_step_async", "godunov", "init_implode",
     "init_diagonal", "init_random", "set_godunov_config", "get_godunov_config", "extents",
+    "conserved_totals", "boundary_flux",
 ]

@@ -14,6 +14,11 @@ Exactly reproduced semantics:
   * step 0 reads the initial ghost primitives, every later step reads ghost
     primitives round-tripped through cons_to_prim (euler_simulation.cpp:50-67);
   * simulated time t = sum of cfl*dt is accumulated in FP64 on the device.
+
+With track_conservation, each step also adds its boundary outflow (the amount by
+which it lowers the interior totals of the conservative fields) to an FP64
+compensated accumulator on the device, so that totals(U_n) + outflow = totals(U_0)
+can be checked to rounding (conservation() reports it).
 """
 from __future__ import annotations
 
@@ -31,7 +36,7 @@ class Simulation:
     the initial state and, after run(), holds the current primitives)."""
 
     def __init__(self, exec_space: ops.ExecSpace, prim: ops.EulerPrimArrays, cons: ops.EulerConsArrays,
-                 eos: ops.PerfectGas, mesh: ops.UniformMesh3d, cfl: float):
+                 eos: ops.PerfectGas, mesh: ops.UniformMesh3d, cfl: float, track_conservation: bool = False):
         self.suf = ops._validate(prim, cons)
         self.ex = exec_space
         self.eos = eos
@@ -54,6 +59,9 @@ class Simulation:
         self._cons_ptrs = ops.cons_ptrs(cons)
         self._started = False
         self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self.track = track_conservation
+        self.outflow = torch.zeros(10, dtype=torch.float64, device=dev)
+        self.totals0: Optional[torch.Tensor] = None
 
     # ------------------------------------------------------------ launches ---
     def _r(self, x):
@@ -74,11 +82,17 @@ class Simulation:
              ctypes.byref(self._cons_ptrs), ctypes.byref(self._ptrs[dst]), n0, n1, n2, self.buf[0].index_bits,
              self._r(self.eos.gamma), self._r(m.dx0), self._r(m.dx1), self._r(m.dx2), self._r(self.cfl),
              self.dt[src:].data_ptr(), self.dt[2:].data_ptr(), self.dt[dst:].data_ptr(), self.time.data_ptr())
+        if self.track:
+            call(f"eb_boundary_flux_{self.suf}", self.ex.context, stream, ctypes.byref(self._ptrs[src]), n0, n1, n2,
+                 self.buf[0].index_bits, self._r(self.eos.gamma), self._r(m.dx0), self._r(m.dx1), self._r(m.dx2),
+                 self.dt[src:].data_ptr(), self._r(self.cfl), self.outflow.data_ptr())
 
     def start(self):
         """First dt from the initial primitives; ghost primitives of the second
         buffer from the ghost conservatives."""
         h = self.ex.handle
+        if self.track:
+            self.totals0 = ops.conserved_totals(self.ex, self.cons)
         ops.time_step_async(self.ex, self.buf[0], self.eos, self.mesh, out=self.dt[0:1])
         self._ghost_refresh(1, h)
         self._started = True
@@ -126,6 +140,18 @@ class Simulation:
             self.ex.stream.wait_stream(s)
             self.graph = g
         return self.graph
+
+    def conservation(self) -> dict:
+        """Interior totals now and at start, the accumulated boundary outflow and
+        the residual totals + outflow - totals0 (FP64, host floats)."""
+        if not self.track or self.totals0 is None:
+            raise RuntimeError("construct with track_conservation=True and run first")
+        now = ops.conserved_totals(self.ex, self.cons)
+        out = self.outflow[0::2] + self.outflow[1::2]
+        t0, t1, o = (x.cpu().tolist() for x in (self.totals0, now, out))
+        fields = ("mass", "energy", "mom0", "mom1", "mom2")
+        res = {f: {"total0": a, "total": b, "outflow": c, "residual": b + c - a} for f, a, b, c in zip(fields, t0, t1, o)}
+        return res
 
     # -------------------------------------------------------------- state ---
     def current_prim(self) -> ops.EulerPrimArrays:

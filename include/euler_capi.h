@@ -213,6 +213,27 @@ int eb_hllc_vjp_f32(hipStream_t stream, int dir, const eb_prim_f32* qL, const eb
                     float gamma);
 
 
+/* ---- conservation diagnostics (FP64, compensated; no reference counterpart) --
+ * Deterministic Neumaier-compensated sums in FP64, for either real type.
+ * eb_conserved_totals: totals_dev[0..4] = sums over the interior cells of
+ *   (d, e, mx0, mx1, mx2).
+ * eb_boundary_flux: the amount B by which one godunov step with
+ *   dt = cfl * dt_dev[0] (as eb_godunov_dev) lowers those totals, i.e.
+ *   dt/dv * sum_d ds_d * (sum of the flux over the high boundary faces - sum over
+ *   the low boundary faces), added to acc_dev[0..9] as (sum, compensation) pairs
+ *   per field. With the frozen ghost layer (godunov.hpp:40-43), totals(U_n) +
+ *   sum of B over the steps equals totals(U_0) up to the update's rounding. */
+int eb_conserved_totals_f64(eb_context* ctx, hipStream_t stream, const eb_cons_f64* cons,
+                            int64_t n0, int64_t n1, int64_t n2, int index_bits, double* totals_dev);
+int eb_conserved_totals_f32(eb_context* ctx, hipStream_t stream, const eb_cons_f32* cons,
+                            int64_t n0, int64_t n1, int64_t n2, int index_bits, double* totals_dev);
+int eb_boundary_flux_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64* prim,
+                         int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                         double dx0, double dx1, double dx2, const double* dt_dev, double cfl, double* acc_dev);
+int eb_boundary_flux_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim,
+                         int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
+                         float dx0, float dx1, float dx2, const float* dt_dev, float cfl, double* acc_dev);
+
 /* ---- batched ensemble (no reference counterpart) ---------------------------
  * Many independent problems, each with its own extents, gamma, cell widths and
  * CFL factor, advanced together: one persistent launch per step over a queue of
