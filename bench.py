@@ -19,6 +19,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
@@ -42,6 +44,7 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline godunov only (for profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-fp32-report", action="store_true", help="skip the FP32-vs-FP64 implosion report")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (HBM counter calibration)")
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
@@ -243,6 +246,87 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
     }
 
 
+def fp32_report(torch, eb, ex, n: int = 512, nsteps: int = 1000, cfl: float = 0.5):
+    """Diagonal implosion (x+y+z > 0.45: rho = p = 1, else rho = 0.125, p = 0.14;
+    u = 0; gamma 1.4; unit cube) at n^3, nsteps steps, FP32 state and arithmetic
+    against FP64 (the FP64 GPU path, bitwise equal to the oracle). Two FP32 runs:
+    (a) its own CFL time steps (fused step loop, graph), (b) driven by the FP64
+    run's time steps rounded to FP32 (reference-order loop), so that both end at
+    the same simulated time. Errors of rho, p and u_k over the interior at t_end;
+    conservation residuals from FP64 compensated totals and boundary outflows."""
+    eos = eb.PerfectGas(1.4)
+    mesh = eb.UniformMesh3d(1.0 / n, 1.0 / n, 1.0 / n)
+
+    def state(dtype):
+        prim = eb.workloads.diagonal_implosion(ex, n, dtype, index_bits=64)
+        cons = eb.EulerConsArrays(*(torch.empty_like(t) for t in prim.tensors()), index_bits=64)
+        eb.prim_to_cons(ex, prim, cons, eos)
+        return prim, cons
+
+    def relres(cons_report):
+        out = {}
+        for name, r in cons_report.items():
+            ref = max(abs(r["total0"]), abs(r["total"]), abs(r["outflow"]))
+            out[name] = {"residual": r["residual"], "relative": r["residual"] / ref if ref else 0.0}
+        return out
+
+    # FP64 reference run, eager, recording the raw dt of every step.
+    t0 = time.perf_counter()
+    prim, cons = state(torch.float64)
+    sim = eb.simulation.Simulation(ex, prim, cons, eos, mesh, cfl, track_conservation=True)
+    tape = torch.empty(nsteps, dtype=torch.float64, device=prim.d.device)
+    sim.start()
+    for k in range(nsteps):
+        tape[k:k + 1].copy_(sim.last_dt())
+        sim.step()
+    sim.finish()
+    t64 = float(sim.time.item())
+    cons64 = relres(sim.conservation())
+    ref = [t.clone() for t in prim.tensors()]
+    del sim, prim, cons
+    torch.cuda.empty_cache()
+
+    def errors(p32):
+        inner = (slice(1, -1),) * 3
+        out = {}
+        for name, a, b in zip(("rho", "p", "u0", "u1", "u2"), p32.tensors(), ref):
+            d = (a[inner].double() - b[inner]).abs()
+            scale = b[inner].abs().mean().item()
+            out[name] = {"L1": d.mean().item(), "Linf": d.max().item(),
+                         "L1_relative": d.mean().item() / scale if scale else None}
+        return out
+
+    # (a) FP32 with its own time steps.
+    prim, cons = state(torch.float32)
+    sim = eb.simulation.Simulation(ex, prim, cons, eos, mesh, cfl, track_conservation=True)
+    sim.run(nsteps, use_graph=True)
+    sim.finish()
+    t32 = float(sim.time.item())
+    own = {"t_end": t32, "errors": errors(prim), "conservation": relres(sim.conservation())}
+    del sim, prim, cons
+    torch.cuda.empty_cache()
+
+    # (b) FP32 driven by the FP64 time steps (reference-order loop, dt on device).
+    prim, cons = state(torch.float32)
+    tape32 = tape.to(torch.float32)
+    hllc = eb.hllc()
+    t_b = 0.0
+    for k in range(nsteps):
+        eb.godunov(ex, prim, cons, eos, mesh, hllc, tape32[k:k + 1], cfl=cfl)
+        eb.cons_to_prim(ex, cons, prim, eos)
+    t_b = float((tape32.double() * float(np.float32(cfl))).sum().item())
+    driven = {"t_end": t_b, "errors": errors(prim)}
+    del prim, cons, ref
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"diagonal implosion {n}^3 (x+y+z>0.45), gamma 1.4, CFL {cfl}, {nsteps} steps",
+        "fp64": {"t_end": t64, "conservation": cons64},
+        "fp32_own_dt": own,
+        "fp32_fp64_dt": driven,
+        "seconds": round(time.perf_counter() - t0, 1),
+    }
+
+
 def profiled_traffic(kernel: str, n: int):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     copy-calibrated) for this kernel and workload, from the committed profile
@@ -369,6 +453,12 @@ def main():
             "speedup_vs_f64": round(ms / ms32, 3),
         }
         del prim32, cons32
+        torch.cuda.empty_cache()
+
+    if not args.quick and not args.no_fp32_report:
+        log("FP32 vs FP64 implosion report")
+        extra["fp32_vs_fp64"] = fp32_report(torch, eb, ex, n=n)
+        log(f"fp32 report: {extra['fp32_vs_fp64']}")
         torch.cuda.empty_cache()
 
     if not args.quick and not args.no_batch:
