@@ -47,23 +47,63 @@ __device__ __forceinline__ T flux_normal(Flux<T> const& f)
     }
 }
 
+// Reciprocal and square root for the derivative arithmetic, which need not round
+// like the forward step (only the branch decisions must match, and those depend
+// on signs and on comparisons of values computed identically on both sides):
+// the hardware estimate refined by two Newton steps, to about 1 ulp.
+__device__ __forceinline__ double fast_rcp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+__device__ __forceinline__ float fast_rcp(float x)
+{
+    return 1.0f / x;
+}
+
+// Per-cell quantities the face derivatives share: the sound speed c and the
+// factors 1/(2p), 1/(2rho) of its derivatives (dc = c (dp/(2p) - drho/(2rho))).
+template <class T>
+struct VjpCell
+{
+    Prim<T> q;
+    T c;
+    T r2p;
+    T r2d;
+};
+
+template <class T>
+__device__ __forceinline__ VjpCell<T> vjp_cell(Prim<T> const& q, T gamma)
+{
+    return {q, sound_speed(gamma, q.d, q.p), T(0.5) * fast_rcp(q.p), T(0.5) * fast_rcp(q.d)};
+}
+
 // Adds (dF/dqL)^T g to gl and (dF/dqR)^T g to gr. Forward values are recomputed
-// in the order of hllc.hpp:61-103.
+// in the order of hllc.hpp:61-103, with the same branch decisions as the flux
+// (the quotients are formed as products with reciprocals, which keeps their
+// signs, so u* > 0 and S_L*S_R > 0 select the same branches).
 template <int Dir, class T>
 __device__ __forceinline__ void hllc_vjp(
-        Prim<T> const& qL,
-        Prim<T> const& qR,
-        T gamma,
+        VjpCell<T> const& L,
+        VjpCell<T> const& R,
         T gm1,
         Flux<T> const& g,
         PrimGrad<T>& gl,
         PrimGrad<T>& gr)
 {
+    Prim<T> const& qL = L.q;
+    Prim<T> const& qR = R.q;
     // ---- forward
     T const unL = normal<Dir>(qL);
     T const unR = normal<Dir>(qR);
-    T const cL = sound_speed(gamma, qL.d, qL.p);
-    T const cR = sound_speed(gamma, qR.d, qR.p);
+    T const cL = L.c;
+    T const cR = R.c;
     bool const sel_c = cL < cR; // max(cL, cR) picks cR
     T const cmax = sel_c ? cR : cL;
     bool const sel_min = unR < unL; // min picks unR
@@ -77,8 +117,8 @@ __device__ __forceinline__ void hllc_vjp(
     T const rcL = qL.d * aL;
     T const rcR = qR.d * aR;
     T const num = qR.p - qL.p + rcL * unL - rcR * unR;
-    T const den = rcL - rcR;
-    T const us = num / den;
+    T const rden = fast_rcp(rcL - rcR);
+    T const us = num * rden;
     T const bL = us - unL;
     T const bR = us - unR;
     T const ps = T(0.5) * (qL.p + qR.p + rcL * bL + rcR * bR);
@@ -91,12 +131,12 @@ __device__ __forceinline__ void hllc_vjp(
     T const uo = sup ? un : us;
     T const po = sup ? q.p : ps;
     T const A = S - un;
-    T const B = S - uo;
-    T const ratio = A / B;
+    T const rB = fast_rcp(S - uo);
+    T const ratio = A * rB;
     T const d_o = ratio * q.d;
     T const C = po * uo - q.p * un;
-    T const D = S - us;
-    T const X = C / D;
+    T const rD = fast_rcp(S - us);
+    T const X = C * rD;
     T const eo = ratio * etot + X;
     T const mass = d_o * uo;
 
@@ -127,8 +167,8 @@ __device__ __forceinline__ void hllc_vjp(
     T g_ratio = g_eo * etot;
     T const g_etot = g_eo * ratio;
     // X = C / D
-    T const g_C = g_eo / D;
-    T const g_D = -g_eo * X / D;
+    T const g_C = g_eo * rD;
+    T const g_D = -g_C * X;
     // C = po*uo - q.p*un
     g_po += g_C * uo;
     g_uo += g_C * po;
@@ -141,8 +181,8 @@ __device__ __forceinline__ void hllc_vjp(
     g_ratio += g_do * q.d;
     gq.d += g_do * ratio;
     // ratio = A / B; A = S - un; B = S - uo
-    T const g_A = g_ratio / B;
-    T const g_B = -g_ratio * ratio / B;
+    T const g_A = g_ratio * rB;
+    T const g_B = -g_A * ratio;
     g_S += g_A + g_B;
     g_un += -g_A;
     g_uo += -g_B;
@@ -195,8 +235,8 @@ __device__ __forceinline__ void hllc_vjp(
     T g_unL = -g_bL;
     T g_unR = -g_bR;
     // us = num / den
-    T const g_num = g_us / den;
-    T const g_den = -g_us * us / den;
+    T const g_num = g_us * rden;
+    T const g_den = -g_num * us;
     // num = (pR - pL) + rcL*unL - rcR*unR; den = rcL - rcR
     g_pR += g_num;
     g_pL -= g_num;
@@ -234,16 +274,32 @@ __device__ __forceinline__ void hllc_vjp(
         g_cL = g_cmax;
     }
     // c = sqrt(gamma*p/d): dc = c/2 * (dp/p - dd/d)
-    g_pL += g_cL * cL / (T(2) * qL.p);
-    g_dL -= g_cL * cL / (T(2) * qL.d);
-    g_pR += g_cR * cR / (T(2) * qR.p);
-    g_dR -= g_cR * cR / (T(2) * qR.d);
+    T const gcL = g_cL * cL;
+    T const gcR = g_cR * cR;
+    g_pL += gcL * L.r2p;
+    g_dL -= gcL * L.r2d;
+    g_pR += gcR * R.r2p;
+    g_dR -= gcR * R.r2d;
     gl.d += g_dL;
     gl.p += g_pL;
     comp<Dir>(gl) += g_unL;
     gr.d += g_dR;
     gr.p += g_pR;
     comp<Dir>(gr) += g_unR;
+}
+
+// Same, from the primitives alone.
+template <int Dir, class T>
+__device__ __forceinline__ void hllc_vjp(
+        Prim<T> const& qL,
+        Prim<T> const& qR,
+        T gamma,
+        T gm1,
+        Flux<T> const& g,
+        PrimGrad<T>& gl,
+        PrimGrad<T>& gr)
+{
+    hllc_vjp<Dir>(vjp_cell(qL, gamma), vjp_cell(qR, gamma), gm1, g, gl, gr);
 }
 
 // J^T g of cons -> prim at the state with primitives q (u = m/rho, p = (gamma-1)(e - |m|^2/(2rho))).

@@ -7,11 +7,11 @@
 //   mu_n = mu_{n+1} + c2p^T( -dtodv_n * R^T(P_n) mu_{n+1}|interior ),
 // and for the first step (P_0 given, U_0 = p2c(P_0)):
 //   dL/dP_0 = -dtodv_0 * R^T(P_0) mu_1|interior + p2c^T mu_1.
-// R^T is evaluated per cell by gathering the vector-Jacobian products of the six
-// faces of the cell: a face between a (low side) and b (high side) carries the
-// weight g = ds_d (mu_a [a interior] - mu_b [b interior]); a receives
-// (dF/dqL)^T g, b receives (dF/dqR)^T g. Each thread owns one cell, so the result
-// is race-free and deterministic.
+// A face between a (low side) and b (high side) carries the weight
+// g = ds_d (mu_a [a interior] - mu_b [b interior]); a receives (dF/dqL)^T g and b
+// receives (dF/dqR)^T g. Each face is differentiated once and each cell's sum is
+// formed by the one thread that owns the cell, so the result is race-free and
+// deterministic.
 #pragma once
 
 #include <cstdint>
@@ -22,71 +22,174 @@
 #include "device_math.hpp"
 #include "kernels.hpp"
 
+#ifndef EB_VJP_MIN_WAVES
+#define EB_VJP_MIN_WAVES 1 // waves per SIMD the tiled adjoint kernel is compiled for (1: no spills)
+#endif
+
 namespace eb {
 
-template <class T, class I>
-__device__ __forceinline__ Flux<T> cons_as_flux(ConsPtrs<T const> const& a, I idx)
+// Keeps the scheduler from interleaving consecutive face derivatives: each one
+// holds ~100 live values, and overlapping two of them would spill.
+__device__ __forceinline__ void sched_fence()
 {
-    return {a.d[idx], a.e[idx], a.mx0[idx], a.mx1[idx], a.mx2[idx]};
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
 }
 
+// Face weight ds * (mu_a [a interior] - mu_b [b interior]), the flags as 0/1 factors.
 template <class T>
 __device__ __forceinline__ Flux<T> face_weight(T ds, Flux<T> const& wa, bool ia, Flux<T> const& wb, bool ib)
 {
-    T const z = T(0);
-    return {ds * ((ia ? wa.d : z) - (ib ? wb.d : z)), ds * ((ia ? wa.e : z) - (ib ? wb.e : z)),
-            ds * ((ia ? wa.m0 : z) - (ib ? wb.m0 : z)), ds * ((ia ? wa.m1 : z) - (ib ? wb.m1 : z)),
-            ds * ((ia ? wa.m2 : z) - (ib ? wb.m2 : z))};
+    T const fa = ia ? T(1) : T(0);
+    T const fb = ib ? T(1) : T(0);
+    return {ds * fma(-fb, wb.d, fa * wa.d), ds * fma(-fb, wb.e, fa * wa.e), ds * fma(-fb, wb.m0, fa * wa.m0),
+            ds * fma(-fb, wb.m1, fa * wa.m1), ds * fma(-fb, wb.m2, fa * wa.m2)};
 }
 
 template <class T>
-__device__ __forceinline__ bool nonzero(Flux<T> const& g)
+__device__ __forceinline__ void add_to(PrimGrad<T>& a, PrimGrad<T> const& b)
 {
-    return g.d != T(0) || g.e != T(0) || g.m0 != T(0) || g.m1 != T(0) || g.m2 != T(0);
+    a.d += b.d;
+    a.p += b.p;
+    a.u0 += b.u0;
+    a.u1 += b.u1;
+    a.u2 += b.u2;
 }
 
-// Adds R^T contributions of the two faces of cell c along Dir to gq.
-template <int Dir, class T, class I>
-__device__ __forceinline__ void gather_faces(
+// Output of one cell from its gathered face adjoints gq (see the header comment).
+template <class T, class I, int Mode>
+__device__ __forceinline__ void vjp_output(ConsPtrs<T> const& mu_out, PrimPtrs<T> const& gprim, I c, Prim<T> const& q,
+                                           Flux<T> const& w, PrimGrad<T> const& gq, T dtodv, T gm1)
+{
+    PrimGrad<T> const gp {-dtodv * gq.d, -dtodv * gq.p, -dtodv * gq.u0, -dtodv * gq.u1, -dtodv * gq.u2};
+    if constexpr (Mode == 0) {
+        Cons<T> const r = c2p_vjp(q, gp, gm1);
+        mu_out.d[c] = w.d + r.d;
+        mu_out.e[c] = w.e + r.e;
+        mu_out.mx0[c] = w.m0 + r.m0;
+        mu_out.mx1[c] = w.m1 + r.m1;
+        mu_out.mx2[c] = w.m2 + r.m2;
+    } else {
+        PrimGrad<T> const r = p2c_vjp(q, Cons<T> {w.d, w.e, w.m0, w.m1, w.m2}, gm1);
+        gprim.d[c] = gp.d + r.d;
+        gprim.p[c] = gp.p + r.p;
+        gprim.ux0[c] = gp.u0 + r.u0;
+        gprim.ux1[c] = gp.u1 + r.u1;
+        gprim.ux2[c] = gp.u2 + r.u2;
+    }
+}
+
+// Ghost cells: each ghost cell on a face of the array (not an edge or corner)
+// shares exactly one face with the interior; its share of that face's derivative
+// is its whole gathered adjoint. Edge and corner ghosts touch no interior cell.
+// Threads take ghost cells first, first + stride, ...
+template <class T, class I, int Mode>
+__device__ __forceinline__ void vjp_ghost_cells(
         PrimPtrs<T const> const& prim,
         ConsPtrs<T const> const& mu,
-        I c,
-        I stride,
-        I pos,
-        I n,
-        bool int_c,
-        bool int_lo,
-        bool int_hi,
-        Prim<T> const& qc,
-        Flux<T> const& wc,
-        T ds,
+        ConsPtrs<T> const& mu_out,
+        PrimPtrs<T> const& gprim,
+        I n0,
+        I n1,
+        I n2,
         T gamma,
         T gm1,
-        PrimGrad<T>& gq)
+        T ds0,
+        T ds1,
+        T ds2,
+        T dtodv,
+        I first,
+        I stride)
 {
-    PrimGrad<T> dummy {T(0), T(0), T(0), T(0), T(0)};
-    if (pos >= 1 && (int_lo || int_c)) {
-        I const cl = c - stride;
-        Flux<T> const g = face_weight(ds, cons_as_flux(mu, cl), int_lo, wc, int_c);
-        if (nonzero(g)) {
-            hllc_vjp<Dir>(load_prim(prim, cl), qc, gamma, gm1, g, dummy, gq);
+    I const nkz = (n2 > 1) ? 2 : 1;
+    I const planes = nkz * n0 * n1;
+    I const inner_k = (n2 > 2) ? n2 - 2 : 0;
+    I const nky = (n1 > 1) ? 2 : 1;
+    I const rows = inner_k * nky * n0;
+    I const inner_j = (n1 > 2) ? n1 - 2 : 0;
+    I const nkx = (n0 > 1) ? 2 : 1;
+    I const cols = inner_k * inner_j * nkx;
+    I const total = planes + rows + cols;
+    auto inside = [](I a, I n) { return a >= 1 && a <= n - 2; };
+    for (I g = first; g < total; g += stride) {
+        I i;
+        I j;
+        I k;
+        if (g < planes) {
+            I const side = g / (n0 * n1);
+            I const r = g % (n0 * n1);
+            i = r % n0;
+            j = r / n0;
+            k = side == 0 ? 0 : n2 - 1;
+        } else if (g < planes + rows) {
+            I const h = g - planes;
+            I const side = h / (inner_k * n0);
+            I const r = h % (inner_k * n0);
+            i = r % n0;
+            k = 1 + r / n0;
+            j = side == 0 ? 0 : n1 - 1;
+        } else {
+            I const h = g - planes - rows;
+            I const side = h / (inner_k * inner_j);
+            I const r = h % (inner_k * inner_j);
+            j = 1 + r % inner_j;
+            k = 1 + r / inner_j;
+            i = side == 0 ? 0 : n0 - 1;
         }
-    }
-    if (pos <= n - 2 && (int_c || int_hi)) {
-        I const ch = c + stride;
-        Flux<T> const g = face_weight(ds, wc, int_c, cons_as_flux(mu, ch), int_hi);
-        if (nonzero(g)) {
-            hllc_vjp<Dir>(qc, load_prim(prim, ch), gamma, gm1, g, gq, dummy);
+        I const c = i + n0 * (j + n1 * k);
+        Prim<T> const q = load_prim(prim, c);
+        Cons<T> const u = load_cons(mu, c);
+        Flux<T> const w {u.d, u.e, u.m0, u.m1, u.m2};
+        PrimGrad<T> gq {T(0), T(0), T(0), T(0), T(0)};
+        PrimGrad<T> dummy {T(0), T(0), T(0), T(0), T(0)};
+        bool const ii = inside(i, n0);
+        bool const jj = inside(j, n1);
+        bool const kk = inside(k, n2);
+        VjpCell<T> const vg = vjp_cell(q, gamma);
+        auto nb = [&](I idx) {
+            Cons<T> const v = load_cons(mu, idx);
+            return Flux<T> {v.d, v.e, v.m0, v.m1, v.m2};
+        };
+        if (jj && kk && (i == 0 || i == n0 - 1) && n0 >= 3) {
+            if (i == 0) {
+                I const b = c + 1;
+                hllc_vjp<0>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds0, w, false, nb(b), true),
+                            gq, dummy);
+            } else {
+                I const a = c - 1;
+                hllc_vjp<0>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds0, nb(a), true, w, false),
+                            dummy, gq);
+            }
+        } else if (ii && kk && (j == 0 || j == n1 - 1) && n1 >= 3) {
+            if (j == 0) {
+                I const b = c + n0;
+                hllc_vjp<1>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds1, w, false, nb(b), true),
+                            gq, dummy);
+            } else {
+                I const a = c - n0;
+                hllc_vjp<1>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds1, nb(a), true, w, false),
+                            dummy, gq);
+            }
+        } else if (ii && jj && (k == 0 || k == n2 - 1) && n2 >= 3) {
+            if (k == 0) {
+                I const b = c + n0 * n1;
+                hllc_vjp<2>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds2, w, false, nb(b), true),
+                            gq, dummy);
+            } else {
+                I const a = c - n0 * n1;
+                hllc_vjp<2>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds2, nb(a), true, w, false),
+                            dummy, gq);
+            }
         }
+        vjp_output<T, I, Mode>(mu_out, gprim, c, q, w, gq, dtodv, gm1);
     }
 }
 
-// Mode 0: mu_out = mu_in + c2p^T(-dtodv * R^T mu_in)           (steps n >= 1)
-// Mode 1: gprim  = -dtodv * R^T mu_in + p2c^T mu_in            (step 0)
 template <class T, class I, int Mode>
-__global__ __launch_bounds__(256) void k_godunov_vjp(
+__global__ __launch_bounds__(256) void k_godunov_vjp_ghost(
         PrimPtrs<T const> prim,
-        ConsPtrs<T const> mu_in,
+        ConsPtrs<T const> mu,
         ConsPtrs<T> mu_out,
         PrimPtrs<T> gprim,
         I n0,
@@ -99,42 +202,243 @@ __global__ __launch_bounds__(256) void k_godunov_vjp(
         T ds2,
         DtArg<T> dta)
 {
-    I const n = n0 * n1 * n2;
-    I const stride = static_cast<I>(gridDim.x) * 256;
-    T const dtodv = resolve_dtodv(dta);
-    for (I c = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); c < n; c += stride) {
-        I const i = c % n0;
-        I const j = (c / n0) % n1;
-        I const k = c / (n0 * n1);
-        auto interior = [&](I a, I b, I e) {
-            return a >= 1 && a <= n0 - 2 && b >= 1 && b <= n1 - 2 && e >= 1 && e <= n2 - 2;
-        };
-        bool const ic = interior(i, j, k);
-        Prim<T> const qc = load_prim(prim, c);
-        Flux<T> const wc = cons_as_flux(mu_in, c);
-        PrimGrad<T> gq {T(0), T(0), T(0), T(0), T(0)};
-        gather_faces<0>(prim, mu_in, c, I(1), i, n0, ic, interior(i - 1, j, k), interior(i + 1, j, k), qc, wc, ds0,
-                        gamma, gm1, gq);
-        gather_faces<1>(prim, mu_in, c, n0, j, n1, ic, interior(i, j - 1, k), interior(i, j + 1, k), qc, wc, ds1,
-                        gamma, gm1, gq);
-        gather_faces<2>(prim, mu_in, c, n0 * n1, k, n2, ic, interior(i, j, k - 1), interior(i, j, k + 1), qc, wc,
-                        ds2, gamma, gm1, gq);
-        PrimGrad<T> const gp {-dtodv * gq.d, -dtodv * gq.p, -dtodv * gq.u0, -dtodv * gq.u1, -dtodv * gq.u2};
-        if constexpr (Mode == 0) {
-            Cons<T> const r = c2p_vjp(qc, gp, gm1);
-            mu_out.d[c] = wc.d + r.d;
-            mu_out.e[c] = wc.e + r.e;
-            mu_out.mx0[c] = wc.m0 + r.m0;
-            mu_out.mx1[c] = wc.m1 + r.m1;
-            mu_out.mx2[c] = wc.m2 + r.m2;
+    vjp_ghost_cells<T, I, Mode>(prim, mu, mu_out, gprim, n0, n1, n2, gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta),
+                                static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x),
+                                static_cast<I>(gridDim.x) * 256);
+}
+
+// Tiled, face-once adjoint step over the interior cells (the forward kernel's
+// geometry: 64 x BY tiles of interior cells, a march along k over `kchunk` planes).
+// Per plane k each thread stages its cell (primitives, c, 1/(2p), 1/(2rho) and the
+// incoming adjoint mu) in LDS and differentiates the x and y faces on its low side
+// and the z face on its high side; two of the waves also take the tile's high-side
+// x and y faces. Each face's two halves go to its two cells: the low cell's half
+// by wave shuffle (x), LDS (y) or registers carried to the next plane (z). Faces
+// between an interior and a ghost cell are included; the ghost cells' own outputs
+// come from the first `ghost_blocks` blocks (vjp_ghost_cells), which recompute the
+// one interior face of each.
+template <class T, class I, int BY, int Mode>
+__global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T const> mu,
+        ConsPtrs<T> mu_out,
+        PrimPtrs<T> gprim,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        int ghost_blocks,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta)
+{
+    if (static_cast<int>(blockIdx.x) < ghost_blocks) {
+        // The first blocks take the ghost cells, so that work overlaps the tiles.
+        I const lt = static_cast<I>(threadIdx.x + 64 * threadIdx.y);
+        vjp_ghost_cells<T, I, Mode>(prim, mu, mu_out, gprim, n0, n1, n2, gamma, gm1, ds0, ds1, ds2,
+                                    resolve_dtodv(dta), static_cast<I>(blockIdx.x) * (64 * BY) + lt,
+                                    static_cast<I>(ghost_blocks) * (64 * BY));
+        return;
+    }
+    constexpr int BX = 64;
+    constexpr int LX = BX + 2;
+    constexpr int LY = BY + 2;
+    constexpr int NHALO = 2 * BX + 2 * BY;
+    constexpr int NF = 13; // d, p, u0, u1, u2, c, 1/(2p), 1/(2rho), mu (5)
+    __shared__ T sq[NF][LY][LX];
+    __shared__ T sgy[5][BY + 1][BX]; // low-cell halves of the y faces: row r = face between tile rows r-1, r
+    __shared__ T sgx[5][BY];         // low-cell halves of the tile's high-side x faces
+
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x) - ghost_blocks, nblk);
+    int const bx = t % tiles_x;
+    int const by = (t / tiles_x) % tiles_y;
+    int const bz = t / (tiles_x * tiles_y);
+    int const tx = threadIdx.x;
+    int const ty = threadIdx.y;
+    int const lid = tx + BX * ty;
+
+    I const i0 = 1 + static_cast<I>(bx) * BX;
+    I const j0 = 1 + static_cast<I>(by) * BY;
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    I const s2 = n0 * n1;
+
+    I const i = i0 + tx;
+    I const j = j0 + ty;
+    auto col_int = [&](I ii) { return ii >= 1 && ii <= n0 - 2; };
+    bool const row_ok = j <= n1 - 2;
+    bool const own_int = col_int(i) && row_ok;
+    I const ic = (i < n0 - 1) ? i : n0 - 1;
+    I const jc = (j < n1 - 1) ? j : n1 - 1;
+    uint32_t const own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
+
+    int hrow = 0;
+    int hcol = 0;
+    uint32_t halo = 0;
+    bool const has_halo = lid < NHALO;
+    if (has_halo) {
+        if (lid < BX) {
+            hrow = 0;
+            hcol = lid + 1;
+        } else if (lid < 2 * BX) {
+            hrow = BY + 1;
+            hcol = lid - BX + 1;
+        } else if (lid < 2 * BX + BY) {
+            hrow = lid - 2 * BX + 1;
+            hcol = 0;
         } else {
-            PrimGrad<T> const r = p2c_vjp(qc, Cons<T> {wc.d, wc.e, wc.m0, wc.m1, wc.m2}, gm1);
-            gprim.d[c] = gp.d + r.d;
-            gprim.p[c] = gp.p + r.p;
-            gprim.ux0[c] = gp.u0 + r.u0;
-            gprim.ux1[c] = gp.u1 + r.u1;
-            gprim.ux2[c] = gp.u2 + r.u2;
+            hrow = lid - 2 * BX - BY + 1;
+            hcol = LX - 1;
         }
+        I hi = i0 - 1 + hcol;
+        I hj = j0 - 1 + hrow;
+        hi = (hi < n0 - 1) ? hi : n0 - 1;
+        hj = (hj < n1 - 1) ? hj : n1 - 1;
+        halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
+    }
+
+    T const dtodv = resolve_dtodv(dta);
+    ConsPtrs<T> const muw {const_cast<T*>(mu.d), const_cast<T*>(mu.e), const_cast<T*>(mu.mx0),
+                           const_cast<T*>(mu.mx1), const_cast<T*>(mu.mx2)};
+    auto load_mu = [&](I plane, uint32_t off) {
+        Cons<T> const u = ld_cons(muw, plane, off);
+        return Flux<T> {u.d, u.e, u.m0, u.m1, u.m2};
+    };
+    auto stage = [&](int r, int c, VjpCell<T> const& v, Flux<T> const& w) {
+        sq[0][r][c] = v.q.d;
+        sq[1][r][c] = v.q.p;
+        sq[2][r][c] = v.q.u0;
+        sq[3][r][c] = v.q.u1;
+        sq[4][r][c] = v.q.u2;
+        sq[5][r][c] = v.c;
+        sq[6][r][c] = v.r2p;
+        sq[7][r][c] = v.r2d;
+        sq[8][r][c] = w.d;
+        sq[9][r][c] = w.e;
+        sq[10][r][c] = w.m0;
+        sq[11][r][c] = w.m1;
+        sq[12][r][c] = w.m2;
+    };
+    auto cell = [&](int r, int c) {
+        return VjpCell<T> {Prim<T> {sq[0][r][c], sq[1][r][c], sq[2][r][c], sq[3][r][c], sq[4][r][c]}, sq[5][r][c],
+                           sq[6][r][c], sq[7][r][c]};
+    };
+    auto weight = [&](int r, int c) { return Flux<T> {sq[8][r][c], sq[9][r][c], sq[10][r][c], sq[11][r][c], sq[12][r][c]}; };
+    PrimGrad<T> const zero {T(0), T(0), T(0), T(0), T(0)};
+
+    // Prologue: the z face below the first plane gives the own cell's first part.
+    VjpCell<T> v_cur = vjp_cell(ld_prim(prim, s2 * kbeg, own), gamma);
+    Flux<T> w_cur = load_mu(s2 * kbeg, own);
+    PrimGrad<T> carry = zero;
+    {
+        VjpCell<T> const vm = vjp_cell(ld_prim(prim, s2 * (kbeg - 1), own), gamma);
+        Flux<T> const wm = load_mu(s2 * (kbeg - 1), own);
+        PrimGrad<T> dummy = zero;
+        hllc_vjp<2>(vm, v_cur, gm1, face_weight(ds2, wm, own_int && kbeg - 1 >= 1, w_cur, own_int), dummy, carry);
+    }
+
+    for (I k = kbeg; k < kend; ++k) {
+        I const plane = s2 * k;
+        Prim<T> const q_next = ld_prim(prim, plane + s2, own);
+        Flux<T> const w_next = load_mu(plane + s2, own);
+        Prim<T> h_q {};
+        Flux<T> h_w {};
+        if (has_halo) {
+            h_q = ld_prim(prim, plane, halo);
+            h_w = load_mu(plane, halo);
+        }
+
+        // A. stage plane k.
+        stage(ty + 1, tx + 1, v_cur, w_cur);
+        if (has_halo) {
+            stage(hrow, hcol, vjp_cell(h_q, gamma), h_w);
+        }
+        lds_barrier();
+
+        // B. face derivatives.
+        PrimGrad<T> gl_x = zero;
+        PrimGrad<T> gq = carry;
+        hllc_vjp<0>(cell(ty + 1, tx), v_cur, gm1,
+                    face_weight(ds0, weight(ty + 1, tx), col_int(i - 1) && row_ok, w_cur, own_int), gl_x, gq);
+        sched_fence();
+        {
+            PrimGrad<T> gl_y = zero;
+            hllc_vjp<1>(cell(ty, tx + 1), v_cur, gm1,
+                        face_weight(ds1, weight(ty, tx + 1), col_int(i) && j - 1 >= 1 && j - 1 <= n1 - 2, w_cur,
+                                    own_int),
+                        gl_y, gq);
+            sgy[0][ty][tx] = gl_y.d;
+            sgy[1][ty][tx] = gl_y.p;
+            sgy[2][ty][tx] = gl_y.u0;
+            sgy[3][ty][tx] = gl_y.u1;
+            sgy[4][ty][tx] = gl_y.u2;
+        }
+        sched_fence();
+        if (ty == BY - 1) {
+            // High-side y faces of the tile: own row and the halo row above it.
+            PrimGrad<T> gl = zero;
+            PrimGrad<T> dummy = zero;
+            hllc_vjp<1>(v_cur, cell(BY + 1, tx + 1), gm1,
+                        face_weight(ds1, w_cur, own_int, weight(BY + 1, tx + 1), col_int(i) && j + 1 <= n1 - 2), gl,
+                        dummy);
+            sgy[0][BY][tx] = gl.d;
+            sgy[1][BY][tx] = gl.p;
+            sgy[2][BY][tx] = gl.u0;
+            sgy[3][BY][tx] = gl.u1;
+            sgy[4][BY][tx] = gl.u2;
+        }
+        sched_fence();
+        if (ty == (BY > 1 ? BY - 2 : 0) && tx < BY) {
+            // High-side x faces of the tile: columns 64 and 65 of row tx+1.
+            int const r = tx + 1;
+            bool const rok = j0 + tx <= n1 - 2;
+            PrimGrad<T> gl = zero;
+            PrimGrad<T> dummy = zero;
+            hllc_vjp<0>(cell(r, BX), cell(r, BX + 1), gm1,
+                        face_weight(ds0, weight(r, BX), col_int(i0 + BX - 1) && rok, weight(r, BX + 1),
+                                    col_int(i0 + BX) && rok),
+                        gl, dummy);
+            sgx[0][tx] = gl.d;
+            sgx[1][tx] = gl.p;
+            sgx[2][tx] = gl.u0;
+            sgx[3][tx] = gl.u1;
+            sgx[4][tx] = gl.u2;
+        }
+        sched_fence();
+        VjpCell<T> const v_next = vjp_cell(q_next, gamma);
+        PrimGrad<T> gr_z = zero;
+        hllc_vjp<2>(v_cur, v_next, gm1, face_weight(ds2, w_cur, own_int, w_next, own_int && k + 1 <= n2 - 2), gq,
+                    gr_z);
+        lds_barrier();
+
+        // C. the low-cell halves from the next lane (x) and the next row (y).
+        PrimGrad<T> hx;
+        hx.d = __shfl_down(gl_x.d, 1, 64);
+        hx.p = __shfl_down(gl_x.p, 1, 64);
+        hx.u0 = __shfl_down(gl_x.u0, 1, 64);
+        hx.u1 = __shfl_down(gl_x.u1, 1, 64);
+        hx.u2 = __shfl_down(gl_x.u2, 1, 64);
+        if (tx == BX - 1) {
+            hx = PrimGrad<T> {sgx[0][ty], sgx[1][ty], sgx[2][ty], sgx[3][ty], sgx[4][ty]};
+        }
+        add_to(gq, hx);
+        add_to(gq, PrimGrad<T> {sgy[0][ty + 1][tx], sgy[1][ty + 1][tx], sgy[2][ty + 1][tx], sgy[3][ty + 1][tx],
+                                sgy[4][ty + 1][tx]});
+        if (own_int) {
+            vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), v_cur.q, w_cur, gq, dtodv,
+                                   gm1);
+        }
+
+        carry = gr_z;
+        v_cur = v_next;
+        w_cur = w_next;
     }
 }
 

@@ -777,10 +777,52 @@ int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu
     eb::ConsPtrs<T> const mo = first ? eb::ConsPtrs<T> {} : cons_ptrs<T>(mu_out);
     eb::PrimPtrs<T> const gp = first ? prim_ptrs<T>(gprim) : eb::PrimPtrs<T> {};
     T const gm1 = gamma - 1;
-    dim3 const grid(pointwise_grid(n));
-#define EB_VJP(I, M)                                                                                              \
-    hipLaunchKernelGGL((eb::k_godunov_vjp<T, I, M>), grid, dim3(256), 0, s, pc, mic, mo, gp, static_cast<I>(n0), \
-                       static_cast<I>(n1), static_cast<I>(n2), gamma, gm1, ds0, ds1, ds2, dta)
+    // Interior cells: the tiled face-once kernel; ghost cells: one face each.
+#ifndef EB_VJP_BY
+#define EB_VJP_BY 4
+#endif
+    constexpr int kBy = EB_VJP_BY;
+    bool const interior = n0 >= 3 && n1 >= 3 && n2 >= 3;
+    int tiles_x = 0;
+    int tiles_y = 0;
+    int tiles_z = 0;
+    int kchunk = 1;
+    if (interior) {
+        if (static_cast<__int128>(n0) * n1 * static_cast<int64_t>(sizeof(T)) >= (static_cast<__int128>(1) << 32)) {
+            return fail(EB_ERR_UNSUPPORTED, "a k-plane must hold fewer than 2^32 bytes per field");
+        }
+        int64_t const columns = ((n0 - 2 + 63) / 64) * ((n1 - 2 + kBy - 1) / kBy);
+        int64_t kc = (n2 - 2) * columns / 1024; // about 1024 blocks
+        kc = kc < 4 ? 4 : (kc > 64 ? 64 : kc);
+        kchunk = static_cast<int>(kc < n2 - 2 ? kc : n2 - 2);
+        int64_t const tz = (n2 - 2 + kchunk - 1) / kchunk;
+        if (columns * tz > std::numeric_limits<int32_t>::max()) {
+            return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
+        }
+        tiles_x = static_cast<int>((n0 - 2 + 63) / 64);
+        tiles_y = static_cast<int>((n1 - 2 + kBy - 1) / kBy);
+        tiles_z = static_cast<int>(tz);
+    }
+    int64_t const inner = interior ? (n0 - 2) * (n1 - 2) * (n2 - 2) : 0;
+    // Ghost cells ride in the first blocks of the tiled launch (a multiple of 8,
+    // so the tiles keep their XCD-contiguous map); without interior, alone.
+    int64_t gb = (n - inner + 64 * kBy - 1) / (64 * kBy);
+    gb = gb > 1024 ? 1024 : gb;
+    int const ghost_blocks = static_cast<int>((gb + 7) / 8 * 8);
+    dim3 const tgrid(static_cast<unsigned int>(ghost_blocks + tiles_x * tiles_y * tiles_z));
+    dim3 const ggrid(reduction_grid(n - inner));
+#define EB_VJP(I, M)                                                                                                 \
+    do {                                                                                                             \
+        if (interior) {                                                                                              \
+            hipLaunchKernelGGL((eb::k_godunov_vjp_tiled<T, I, kBy, M>), tgrid, dim3(64, kBy), 0, s, pc, mic, mo, gp, \
+                               static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y, \
+                               tiles_z, ghost_blocks, gamma, gm1, ds0, ds1, ds2, dta);                               \
+        } else {                                                                                                     \
+            hipLaunchKernelGGL((eb::k_godunov_vjp_ghost<T, I, M>), ggrid, dim3(256), 0, s, pc, mic, mo, gp,           \
+                               static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma, gm1, ds0, ds1, ds2, \
+                               dta);                                                                                 \
+        }                                                                                                            \
+    } while (0)
     if (bits == 32) {
         if (first) {
             EB_VJP(int32_t, 1);
