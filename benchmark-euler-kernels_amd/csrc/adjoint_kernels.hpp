@@ -37,6 +37,18 @@ __device__ __forceinline__ void sched_fence()
 #endif
 }
 
+template <class T>
+__device__ __forceinline__ bool nonzero(Flux<T> const& w)
+{
+    return w.d != T(0) || w.e != T(0) || w.m0 != T(0) || w.m1 != T(0) || w.m2 != T(0);
+}
+
+template <class T>
+__device__ __forceinline__ bool nonzero(PrimGrad<T> const& g)
+{
+    return g.d != T(0) || g.p != T(0) || g.u0 != T(0) || g.u1 != T(0) || g.u2 != T(0);
+}
+
 // Face weight ds * (mu_a [a interior] - mu_b [b interior]), the flags as 0/1 factors.
 template <class T>
 __device__ __forceinline__ Flux<T> face_weight(T ds, Flux<T> const& wa, bool ia, Flux<T> const& wb, bool ib)
@@ -355,12 +367,26 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
             h_w = load_mu(plane, halo);
         }
 
-        // A. stage plane k.
+        // A. stage plane k. Where the incoming adjoint and the carried z-face part
+        // vanish over the whole tile, its ring and the plane above, every face
+        // weight of the plane is zero: the face derivatives are skipped (the
+        // adjoint of a localised loss is zero outside its domain of dependence).
         stage(ty + 1, tx + 1, v_cur, w_cur);
+        bool active = nonzero(w_cur) || nonzero(w_next) || nonzero(carry);
         if (has_halo) {
             stage(hrow, hcol, vjp_cell(h_q, gamma), h_w);
+            active = active || nonzero(h_w);
         }
-        lds_barrier();
+        if (__syncthreads_or(active ? 1 : 0) == 0) {
+            if (own_int) {
+                vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), v_cur.q, w_cur, carry,
+                                       dtodv, gm1);
+            }
+            carry = zero;
+            v_cur = vjp_cell(q_next, gamma);
+            w_cur = w_next;
+            continue;
+        }
 
         // B. face derivatives.
         PrimGrad<T> gl_x = zero;
