@@ -76,8 +76,8 @@ def _signatures():
         "eb_batch_create_f64": [_P(_vp), _P(BatchInstanceF64), _int, _int],
         "eb_batch_create_f32": [_P(_vp), _P(BatchInstanceF32), _int, _int],
         "eb_batch_destroy": [_vp],
-        "eb_batch_set_tiles": [_vp, _int, _int],
-        "eb_batch_get_tiles": [_vp, _P(_int), _P(_int), _P(_int), _P(_int)],
+        "eb_batch_set_tiles": [_vp, _int, _int, _int],
+        "eb_batch_get_tiles": [_vp, _P(_int), _P(_int), _P(_int), _P(_int), _P(_int)],
         "eb_batch_device_state": [_vp, _P(_vp), _P(_vp)],
         "eb_batch_copy_state": [_vp, _vp, _vp, _vp],
         "eb_batch_start": [_vp, _vp],

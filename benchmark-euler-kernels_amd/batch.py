@@ -63,15 +63,17 @@ class Ensemble:
         self.per_instance_graph: Optional[torch.cuda.CUDAGraph] = None
 
     # ------------------------------------------------------------- config ---
-    def set_tiles(self, by: int, kchunk: int) -> None:
-        call("eb_batch_set_tiles", self._handle, by, kchunk)
+    def set_tiles(self, by: int, kchunk: int, layout: int = 0) -> None:
+        """Tile layout 0: 64 x by columns (by 4, 8, 16); 1: runs of 64*by cells of a
+        plane (by 4, 8; every n0 <= 64*by). kchunk planes per tile, 0 = whole columns."""
+        call("eb_batch_set_tiles", self._handle, layout, by, kchunk)
         self.graph = self.per_instance_graph = None
 
     def tiles(self):
-        by, kc, nt, grid = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        call("eb_batch_get_tiles", self._handle, ctypes.byref(by), ctypes.byref(kc), ctypes.byref(nt),
-             ctypes.byref(grid))
-        return {"by": by.value, "kchunk": kc.value, "tiles": nt.value, "grid": grid.value}
+        lay, by, kc, nt, grid = (ctypes.c_int() for _ in range(5))
+        call("eb_batch_get_tiles", self._handle, ctypes.byref(lay), ctypes.byref(by), ctypes.byref(kc),
+             ctypes.byref(nt), ctypes.byref(grid))
+        return {"layout": lay.value, "by": by.value, "kchunk": kc.value, "tiles": nt.value, "grid": grid.value}
 
     def autotune(self, reps: int = 3) -> float:
         """Times the candidate tile shapes on the live state (left unchanged) and

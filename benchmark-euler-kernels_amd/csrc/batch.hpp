@@ -5,6 +5,10 @@
 // godunov_tile in kernels.hpp), so every instance's results are bitwise those of a
 // launch of its own.
 //
+// Tiles are either 64 x BY columns of cells (layout 0) or runs of 64*BY
+// consecutive cells of a plane (layout 1, godunov_tile_lin), which lose no lanes
+// to ragged x extents.
+//
 // Scheduling: the tile list is split into one contiguous range per XCD (ranges of
 // about equal cell count, instances kept together so halo re-reads stay in that
 // XCD's L2); block b serves queue b % nq and pulls tiles with an atomic counter. The
@@ -79,7 +83,7 @@ __device__ __forceinline__ int wave_uniform(int v)
 // One step of every instance. tune_dt != nullptr times the launch on live state:
 // dt is read from tune_dt (zeros), the conservatives are stored back unchanged,
 // the next dt lands in tune_out and no simulated time is accumulated.
-template <class T, class I, int BY>
+template <class T, class I, int BY, int Layout>
 __global__ __launch_bounds__(64 * BY) void k_batch_step(
         BatchInstance<T> const* insts,
         BatchTile const* tiles,
@@ -117,19 +121,29 @@ __global__ __launch_bounds__(64 * BY) void k_batch_step(
         }
         BatchTile const tl = tiles[t];
         BatchInstance<T> const& in = insts[tl.inst];
-        int const bx = tl.local % in.tiles_x;
-        int const by = (tl.local / in.tiles_x) % in.tiles_y;
-        int const bz = tl.local / (in.tiles_x * in.tiles_y);
         PrimPtrs<T> const src = in.buf[parity];
         PrimPtrs<T const> const pin {src.d, src.p, src.ux0, src.ux1, src.ux2};
         T const* dt_in = tune ? tune_dt + tl.inst : in.dt + parity;
         // dt/dv exactly as for a single grid (euler_simulation.cpp:65, godunov.hpp:32)
         T const dt = in.cfl * dt_in[0];
         T const dtodv = dt / in.dv;
-        T const lm = godunov_tile<T, I, BY, true>(pin, in.cons, in.buf[1 - parity], static_cast<I>(in.n0),
-                                                   static_cast<I>(in.n1), static_cast<I>(in.n2), in.kchunk, bx, by,
-                                                   bz, in.gamma, in.gm1, in.ds0, in.ds1, in.ds2, dtodv, in.invdx0,
-                                                   in.invdx1, in.invdx2, !tune);
+        T lm;
+        if constexpr (Layout == 1) {
+            // linear tiles: tiles_x tiles per plane, tiles_y = 1
+            lm = godunov_tile_lin<T, I, BY, true>(pin, in.cons, in.buf[1 - parity], static_cast<I>(in.n0),
+                                                  static_cast<I>(in.n1), static_cast<I>(in.n2), in.kchunk,
+                                                  tl.local % in.tiles_x, tl.local / in.tiles_x, in.gamma, in.gm1,
+                                                  in.ds0, in.ds1, in.ds2, dtodv, in.invdx0, in.invdx1, in.invdx2,
+                                                  !tune);
+        } else {
+            int const bx = tl.local % in.tiles_x;
+            int const by = (tl.local / in.tiles_x) % in.tiles_y;
+            int const bz = tl.local / (in.tiles_x * in.tiles_y);
+            lm = godunov_tile<T, I, BY, true>(pin, in.cons, in.buf[1 - parity], static_cast<I>(in.n0),
+                                              static_cast<I>(in.n1), static_cast<I>(in.n2), in.kchunk, bx, by, bz,
+                                              in.gamma, in.gm1, in.ds0, in.ds1, in.ds2, dtodv, in.invdx0, in.invdx1,
+                                              in.invdx2, !tune);
+        }
         T const bm = block_min<T, BY>(lm);
         if (lid == 0) {
             DtSink<T> const sink {in.scratch, tune ? tune_out + tl.inst : in.dt + (1 - parity), in.dt + 2,

@@ -16,6 +16,7 @@ struct eb_batch
     int real_bits = 64; // 64: double, 32: float
     int index_bits = 64;
     int ninst = 0;
+    int layout = 0; // 0: 64 x by tiles, 1: linear runs of 64*by cells
     int by = 8;
     int kchunk = 32;
     int ntiles = 0;
@@ -45,9 +46,9 @@ using inst_c = std::conditional_t<std::is_same_v<T, double>, eb_batch_instance_f
 
 constexpr int kBatchQueues = 8; // one per XCD
 
-bool batch_by_supported(int by)
+bool batch_by_supported(int layout, int by)
 {
-    return by == 4 || by == 8 || by == 16;
+    return layout == 0 ? (by == 4 || by == 8 || by == 16) : (layout == 1 && (by == 4 || by == 8));
 }
 
 template <class T>
@@ -67,7 +68,7 @@ void batch_free(eb_batch* b)
     }
 }
 
-template <class T, class I, int BY>
+template <class T, class I, int BY, int L>
 int batch_resident_blocks(int* out)
 {
     int dev = 0;
@@ -75,7 +76,7 @@ int batch_resident_blocks(int* out)
     int per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess
         || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
-        || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eb::k_batch_step<T, I, BY>, 64 * BY, 0)
+        || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eb::k_batch_step<T, I, BY, L>, 64 * BY, 0)
                    != hipSuccess) {
         return fail(EB_ERR_HIP, "occupancy query failed");
     }
@@ -84,22 +85,25 @@ int batch_resident_blocks(int* out)
 }
 
 template <class T, class I>
-int batch_resident(int by, int* out)
+int batch_resident(int layout, int by, int* out)
 {
+    if (layout == 1) {
+        return by == 4 ? batch_resident_blocks<T, I, 4, 1>(out) : batch_resident_blocks<T, I, 8, 1>(out);
+    }
     if (by == 4) {
-        return batch_resident_blocks<T, I, 4>(out);
+        return batch_resident_blocks<T, I, 4, 0>(out);
     }
     if (by == 8) {
-        return batch_resident_blocks<T, I, 8>(out);
+        return batch_resident_blocks<T, I, 8, 0>(out);
     }
-    return batch_resident_blocks<T, I, 16>(out);
+    return batch_resident_blocks<T, I, 16, 0>(out);
 }
 
 // Tiles of every instance for tile height `by` and `kchunk` planes per tile (0: a
 // whole column), split into queues of about equal cell count; uploads descriptors,
 // tiles and ranges (blocking; configuration time only).
 template <class T>
-int batch_layout(eb_batch* b, int by, int kchunk)
+int batch_layout(eb_batch* b, int layout, int by, int kchunk)
 {
     eb::BatchInstance<T>* hi = batch_host<T>(b);
     std::vector<eb::BatchTile> tiles;
@@ -111,8 +115,16 @@ int batch_layout(eb_batch* b, int by, int kchunk)
         int64_t const nz = in.n2 - 2;
         int64_t const kc = kchunk > 0 ? std::min<int64_t>(kchunk, nz) : nz;
         in.kchunk = static_cast<int>(kc);
-        in.tiles_x = static_cast<int>((nx + 63) / 64);
-        in.tiles_y = static_cast<int>((ny + by - 1) / by);
+        if (layout == 1) {
+            if (in.n0 > 64 * by) {
+                return fail(EB_ERR_UNSUPPORTED, "linear tiles need rows of at most 64*by cells");
+            }
+            in.tiles_x = static_cast<int>((in.n0 * ny + 64 * by - 1) / (64 * by));
+            in.tiles_y = 1;
+        } else {
+            in.tiles_x = static_cast<int>((nx + 63) / 64);
+            in.tiles_y = static_cast<int>((ny + by - 1) / by);
+        }
         in.tiles_z = static_cast<int>((nz + kc - 1) / kc);
         int64_t const nt = int64_t(in.tiles_x) * in.tiles_y * in.tiles_z;
         if (nt + static_cast<int64_t>(tiles.size()) > std::numeric_limits<int32_t>::max() / 2) {
@@ -120,20 +132,26 @@ int batch_layout(eb_batch* b, int by, int kchunk)
         }
         in.ntiles = static_cast<int>(nt);
         for (int t = 0; t < in.ntiles; ++t) {
-            int const bx = t % in.tiles_x;
-            int const byy = (t / in.tiles_x) % in.tiles_y;
             int const bz = t / (in.tiles_x * in.tiles_y);
-            double const w = static_cast<double>(std::min<int64_t>(64, nx - 64 * int64_t(bx)))
-                             * static_cast<double>(std::min<int64_t>(by, ny - int64_t(by) * byy))
-                             * static_cast<double>(std::min<int64_t>(kc, nz - kc * bz));
+            double const planes = static_cast<double>(std::min<int64_t>(kc, nz - kc * bz));
+            double w;
+            if (layout == 1) {
+                int const tp = t % in.tiles_x;
+                w = static_cast<double>(std::min<int64_t>(64 * by, in.n0 * ny - int64_t(64 * by) * tp)) * planes;
+            } else {
+                int const bx = t % in.tiles_x;
+                int const byy = (t / in.tiles_x) % in.tiles_y;
+                w = static_cast<double>(std::min<int64_t>(64, nx - 64 * int64_t(bx)))
+                    * static_cast<double>(std::min<int64_t>(by, ny - int64_t(by) * byy)) * planes;
+            }
             tiles.push_back({i, t});
             cost.push_back(w);
         }
     }
     int const ntiles = static_cast<int>(tiles.size());
     int resident = 0;
-    int const st = b->index_bits == 32 ? batch_resident<T, int32_t>(by, &resident)
-                                       : batch_resident<T, int64_t>(by, &resident);
+    int const st = b->index_bits == 32 ? batch_resident<T, int32_t>(layout, by, &resident)
+                                       : batch_resident<T, int64_t>(layout, by, &resident);
     if (st != EB_OK) {
         return st;
     }
@@ -175,6 +193,7 @@ int batch_layout(eb_batch* b, int by, int kchunk)
                    != hipSuccess) {
         return fail(EB_ERR_HIP, "batch layout upload failed");
     }
+    b->layout = layout;
     b->by = by;
     b->kchunk = kchunk;
     b->ntiles = ntiles;
@@ -268,7 +287,7 @@ int batch_create(eb_batch** out, inst_c<T> const* inst, int n, int index_bits)
         hi[i].time = b->time + i;
         hi[i].scratch = b->scratch + i;
     }
-    int const st = batch_layout<T>(b, 8, 32);
+    int const st = batch_layout<T>(b, 0, 8, 32);
     if (st != EB_OK) {
         batch_free(b);
         delete b;
@@ -331,16 +350,23 @@ int batch_launch(eb_batch* b, hipStream_t s, int parity, bool tune)
     T const* tdt = tune ? static_cast<T const*>(b->tune_dt) : nullptr;
     T* tout = tune ? static_cast<T*>(b->tune_out) : nullptr;
     dim3 const grid(static_cast<unsigned int>(std::max(b->grid, 1)));
-    if (b->by == 4) {
-        hipLaunchKernelGGL((eb::k_batch_step<T, I, 4>), grid, dim3(64, 4), 0, s, insts, b->tiles, b->ranges, b->nq,
-                           b->queues, parity, tdt, tout);
+#define EB_BATCH_LAUNCH(BY, L)                                                                                    \
+    hipLaunchKernelGGL((eb::k_batch_step<T, I, BY, L>), grid, dim3(64, BY), 0, s, insts, b->tiles, b->ranges, \
+                       b->nq, b->queues, parity, tdt, tout)
+    if (b->layout == 1) {
+        if (b->by == 4) {
+            EB_BATCH_LAUNCH(4, 1);
+        } else {
+            EB_BATCH_LAUNCH(8, 1);
+        }
+    } else if (b->by == 4) {
+        EB_BATCH_LAUNCH(4, 0);
     } else if (b->by == 8) {
-        hipLaunchKernelGGL((eb::k_batch_step<T, I, 8>), grid, dim3(64, 8), 0, s, insts, b->tiles, b->ranges, b->nq,
-                           b->queues, parity, tdt, tout);
+        EB_BATCH_LAUNCH(8, 0);
     } else {
-        hipLaunchKernelGGL((eb::k_batch_step<T, I, 16>), grid, dim3(64, 16), 0, s, insts, b->tiles, b->ranges,
-                           b->nq, b->queues, parity, tdt, tout);
+        EB_BATCH_LAUNCH(16, 0);
     }
+#undef EB_BATCH_LAUNCH
     return check_launch("batch_step");
 }
 
@@ -372,20 +398,30 @@ int batch_per_instance(eb_batch* b, hipStream_t s, int parity)
 template <class T, class I>
 int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best_ms)
 {
-    static constexpr int kBy[] = {4, 8, 16};
+    static constexpr int kShapes[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 4}, {1, 8}}; // (layout, by)
     static constexpr int kKc[] = {0, 64, 32, 16, 8};
+    int64_t max_n0 = 0;
+    for (int i = 0; i < b->ninst; ++i) {
+        max_n0 = std::max<int64_t>(max_n0, batch_host<T>(b)[i].n0);
+    }
     hipEvent_t e0 = nullptr;
     hipEvent_t e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         return fail(EB_ERR_HIP, "event creation failed");
     }
+    int best_layout = b->layout;
     int best_by = b->by;
     int best_kc = b->kchunk;
     float best = -1.0f;
     int st = EB_OK;
-    for (int by : kBy) {
+    for (auto const& shape : kShapes) {
+        int const layout = shape[0];
+        int const by = shape[1];
+        if (layout == 1 && max_n0 > 64 * by) {
+            continue;
+        }
         for (int kc : kKc) {
-            st = batch_layout<T>(b, by, kc);
+            st = batch_layout<T>(b, layout, by, kc);
             if (st != EB_OK) {
                 break;
             }
@@ -407,6 +443,7 @@ int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best
             ms /= static_cast<float>(reps);
             if (best < 0.0f || ms < best) {
                 best = ms;
+                best_layout = layout;
                 best_by = by;
                 best_kc = kc;
             }
@@ -417,7 +454,7 @@ int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    int const st2 = batch_layout<T>(b, best_by, best_kc);
+    int const st2 = batch_layout<T>(b, best_layout, best_by, best_kc);
     if (st != EB_OK) {
         return st;
     }
@@ -485,22 +522,33 @@ int eb_batch_destroy(eb_batch* batch)
     return EB_OK;
 }
 
-int eb_batch_set_tiles(eb_batch* batch, int by, int kchunk)
+int eb_batch_set_tiles(eb_batch* batch, int layout, int by, int kchunk)
 {
     if (batch == nullptr || kchunk < 0) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null batch or negative kchunk");
     }
-    if (!batch_by_supported(by)) {
-        return fail(EB_ERR_UNSUPPORTED, "tile height must be 4, 8 or 16");
+    if (!batch_by_supported(layout, by)) {
+        return fail(EB_ERR_UNSUPPORTED, "tiles: layout 0 with by 4, 8 or 16; layout 1 with by 4 or 8");
     }
-    return batch->real_bits == 64 ? batch_layout<double>(batch, by, kchunk) : batch_layout<float>(batch, by, kchunk);
+    int const keep_layout = batch->layout;
+    int const keep_by = batch->by;
+    int const keep_kc = batch->kchunk;
+    int const st = batch->real_bits == 64 ? batch_layout<double>(batch, layout, by, kchunk)
+                                          : batch_layout<float>(batch, layout, by, kchunk);
+    if (st != EB_OK) {
+        // leave the previous, valid layout in place
+        (void)(batch->real_bits == 64 ? batch_layout<double>(batch, keep_layout, keep_by, keep_kc)
+                                      : batch_layout<float>(batch, keep_layout, keep_by, keep_kc));
+    }
+    return st;
 }
 
-int eb_batch_get_tiles(const eb_batch* batch, int* by, int* kchunk, int* ntiles, int* grid)
+int eb_batch_get_tiles(const eb_batch* batch, int* layout, int* by, int* kchunk, int* ntiles, int* grid)
 {
-    if (batch == nullptr || by == nullptr || kchunk == nullptr) {
+    if (batch == nullptr || layout == nullptr || by == nullptr || kchunk == nullptr) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
     }
+    *layout = batch->layout;
     *by = batch->by;
     *kchunk = batch->kchunk;
     if (ntiles != nullptr) {

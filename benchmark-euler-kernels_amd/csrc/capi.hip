@@ -35,7 +35,7 @@ std::atomic<int> g_tile_kchunk {0}; // 0: sized per launch by auto_kchunk
 bool config_supported(int variant, int by)
 {
     return (variant == 0 && (by == 2 || by == 4)) || (variant == 1 && (by == 4 || by == 8 || by == 16))
-           || (variant == 2 && (by == 4 || by == 8));
+           || (variant == 2 && (by == 4 || by == 8)) || (variant == 3 && (by == 4 || by == 8));
 }
 
 int fail(int status, std::string const& msg)
@@ -222,7 +222,13 @@ void launch_tiled(
     int const tiles_y = static_cast<int>((n1 - 2 + BY - 1) / BY);
     int const tiles_z = static_cast<int>((n2 - 2 + kchunk - 1) / kchunk);
     int const nblk = tiles_x * tiles_y * tiles_z;
-    if constexpr (Variant == 2) {
+    if constexpr (Variant == 3) {
+        int const tiles_plane = static_cast<int>((n0 * (n1 - 2) + 64 * BY - 1) / (64 * BY));
+        hipLaunchKernelGGL((eb::k_godunov_linear<T, I, BY, Fused>), dim3(tiles_plane * tiles_z), dim3(64, BY), 0, s,
+                           pc, cu, pout, static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk,
+                           tiles_plane, tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1,
+                           invdx2, sink);
+    } else if constexpr (Variant == 2) {
         hipLaunchKernelGGL((eb::k_godunov_stream<T, I, BY, Fused>), dim3(nblk), dim3(64, BY + 1), 0, s, pc, cu, pout,
                            static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
                            tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
@@ -277,7 +283,9 @@ int dispatch_tiled(
     int const kc_set = g_tile_kchunk.load();
     int const kchunk = kc_set > 0 ? kc_set : auto_kchunk(n0, n1, n2, by);
     // The block count must fit a 32-bit grid and a plane must fit 32-bit offsets.
-    int64_t const tiles = ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by) * ((n2 - 2 + kchunk - 1) / kchunk);
+    int64_t const columns = variant == 3 ? (n0 * (n1 - 2) + 64 * by - 1) / (64 * by)
+                                         : ((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by);
+    int64_t const tiles = columns * ((n2 - 2 + kchunk - 1) / kchunk);
     if (tiles > std::numeric_limits<int32_t>::max()) {
         return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
     }
@@ -290,7 +298,14 @@ int dispatch_tiled(
     if (variant == 2 && !aligned16(pc)) {
         return fail(EB_ERR_UNSUPPORTED, "the streamed variant needs 16-byte aligned primitive fields");
     }
-    if (variant == 2 && by == 4) {
+    if (variant == 3 && n0 > 64 * by) {
+        return fail(EB_ERR_UNSUPPORTED, "linear tiles need rows of at most 64*by cells (n0 <= 64*by)");
+    }
+    if (variant == 3 && by == 4) {
+        EB_LAUNCH(3, 4);
+    } else if (variant == 3 && by == 8) {
+        EB_LAUNCH(3, 8);
+    } else if (variant == 2 && by == 4) {
         EB_LAUNCH(2, 4);
     } else if (variant == 2 && by == 8) {
         EB_LAUNCH(2, 8);
@@ -531,7 +546,7 @@ int eb_set_godunov_config(int variant, int by, int kchunk)
         return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 0 (0: automatic)");
     }
     if (!config_supported(variant, by)) {
-        return fail(EB_ERR_UNSUPPORTED, "unsupported (variant, tile height): (0, 2|4) or (1, 4|8|16)");
+        return fail(EB_ERR_UNSUPPORTED, "unsupported (variant, tile height): (0, 2|4), (1, 4|8|16), (2, 4|8), (3, 4|8)");
     }
     g_variant.store(variant);
     g_tile_by.store(by);

@@ -609,6 +609,202 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     }
 }
 
+// Linear-tile Godunov update (2.5-D march along k), configuration variant 3.
+//
+// A block of NW waves owns 64*NW consecutive cells of a k-plane in array order,
+// a = i + n0*j, taken from the interior rows j = 1 .. n1-2 (ghost columns i = 0
+// and n0-1 ride along and produce nothing). Rows wrap inside the tile, so no lane
+// is lost to a ragged x extent. Per plane k:
+//   A. each thread stages its cell (primitives, c, E); the block also stages the
+//      row below its first cell and the row above its last (2*n0 halo cells);
+//   B. each thread solves the x face and the y face on its low side (from the
+//      staged neighbours a-1 and a-n0) and the z face on its high side; the y
+//      faces above the last n0 cells and the x face after the last cell are
+//      solved once more by the first threads;
+//   C. each thread combines its faces in the reference's order (x, y, z;
+//      godunov.hpp:48-82) and updates its cell in place.
+// As in the 64 x BY tiles, every face is solved once with the operands of the
+// reference's second solve of it, so results are bitwise the reference order's.
+// Requires n0 <= 64*NW, n0, n1, n2 >= 3 and n0*n1*sizeof(T) < 2^32.
+template <class T, class I, int NW, bool Fused>
+__device__ __forceinline__ T godunov_tile_lin(
+        PrimPtrs<T const> const& prim,
+        ConsPtrs<T> const& cons,
+        PrimPtrs<T> const& prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tile,
+        int bz,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        T dtodv,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        bool apply)
+{
+    constexpr int NT = 64 * NW;     // cells per tile
+    constexpr int LS = 3 * NT;      // staged cells: n0 below + NT + n0 above, n0 <= NT
+    constexpr int NF = 7;           // d, p, u0, u1, u2, c, E
+    __shared__ T sq[NF][LS];
+    __shared__ T sfx[5][NT + 1];    // x face on the low side of tile cell t; [NT]: after the last cell
+    __shared__ T sfy[5][NT];        // y face on the low side of tile cell t
+    __shared__ T sfe[5][NT];        // y faces above tile cells NT-n0 .. NT-1
+
+    int const t = threadIdx.x + 64 * threadIdx.y;
+    I const s2 = n0 * n1;
+    I const a_begin = n0;            // first cell of row 1
+    I const a_end = n0 * (n1 - 1);   // one past the last cell of row n1-2
+    I const A = a_begin + static_cast<I>(tile) * NT;
+    I const a = A + t;
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    int const w0 = static_cast<int>(n0); // staged offset of tile cell 0
+
+    I const i = a % n0;
+    bool const valid = a < a_end && i >= 1 && i <= n0 - 2;
+    auto clampa = [&](I x) { return x < s2 - 1 ? x : s2 - 1; };
+    uint32_t const own = static_cast<uint32_t>(clampa(a)) * static_cast<uint32_t>(sizeof(T));
+
+    T local_min = Limits<T>::max();
+    auto stage = [&](int r, Prim<T> const& q, T cs, T e) {
+        sq[0][r] = q.d;
+        sq[1][r] = q.p;
+        sq[2][r] = q.u0;
+        sq[3][r] = q.u1;
+        sq[4][r] = q.u2;
+        sq[5][r] = cs;
+        sq[6][r] = e;
+    };
+    auto cell = [&](int r) { return Prim<T> {sq[0][r], sq[1][r], sq[2][r], sq[3][r], sq[4][r]}; };
+    auto put = [&](auto& arr, int r, Flux<T> const& f) {
+        arr[0][r] = f.d;
+        arr[1][r] = f.e;
+        arr[2][r] = f.m0;
+        arr[3][r] = f.m1;
+        arr[4][r] = f.m2;
+    };
+
+    Flux<T> fz_lo;
+    Prim<T> q_cur = ld_prim(prim, s2 * kbeg, own);
+    T c_cur = sound_speed(gamma, q_cur.d, q_cur.p);
+    T e_cur = total_energy(q_cur, gm1);
+    {
+        Prim<T> const qm = ld_prim(prim, s2 * (kbeg - 1), own);
+        fz_lo = hllc_flux<2>(qm, sound_speed(gamma, qm.d, qm.p), total_energy(qm, gm1), q_cur, c_cur, e_cur);
+    }
+
+    for (I k = kbeg; k < kend; ++k) {
+        I const plane = s2 * k;
+        Prim<T> const q_next = ld_prim(prim, plane + s2, own);
+        Cons<T> u = ld_cons(cons, plane, own);
+
+        // A. stage plane k: own cell, then the halo rows (2*n0 cells, h < n0 below).
+        stage(w0 + t, q_cur, c_cur, e_cur);
+        for (int h = t; h < 2 * w0; h += NT) {
+            I const b = h < w0 ? A - n0 + h : A + NT + (h - w0);
+            Prim<T> const q = ld_prim(prim, plane, static_cast<uint32_t>(clampa(b)) * static_cast<uint32_t>(sizeof(T)));
+            stage(h < w0 ? h : w0 + NT + (h - w0), q, sound_speed(gamma, q.d, q.p), total_energy(q, gm1));
+        }
+        lds_barrier();
+
+        // B. faces.
+        T const c_next = sound_speed(gamma, q_next.d, q_next.p);
+        T const e_next = total_energy(q_next, gm1);
+        int const r = w0 + t;
+        Flux<T> const fx_lo = hllc_flux<0>(cell(r - 1), sq[5][r - 1], sq[6][r - 1], q_cur, c_cur, e_cur);
+        Flux<T> const fy_lo = hllc_flux<1>(cell(r - w0), sq[5][r - w0], sq[6][r - w0], q_cur, c_cur, e_cur);
+        Flux<T> const fz_hi = hllc_flux<2>(q_cur, c_cur, e_cur, q_next, c_next, e_next);
+        put(sfx, t, fx_lo);
+        put(sfy, t, fy_lo);
+        if (t < w0) {
+            // y face above tile cell NT - n0 + t
+            int const lo = w0 + NT - w0 + t; // staged index of that cell
+            Flux<T> const f
+                    = hllc_flux<1>(cell(lo), sq[5][lo], sq[6][lo], cell(lo + w0), sq[5][lo + w0], sq[6][lo + w0]);
+            put(sfe, t, f);
+        }
+        if (t == NT - 1) {
+            Flux<T> const f = hllc_flux<0>(q_cur, c_cur, e_cur, cell(r + 1), sq[5][r + 1], sq[6][r + 1]);
+            sfx[0][NT] = f.d;
+            sfx[1][NT] = f.e;
+            sfx[2][NT] = f.m0;
+            sfx[3][NT] = f.m1;
+            sfx[4][NT] = f.m2;
+        }
+        lds_barrier();
+
+        // C. combine and update.
+        Flux<T> const fx_hi {sfx[0][t + 1], sfx[1][t + 1], sfx[2][t + 1], sfx[3][t + 1], sfx[4][t + 1]};
+        int const up = t + w0;
+        Flux<T> const fy_hi = up < NT ? Flux<T> {sfy[0][up], sfy[1][up], sfy[2][up], sfy[3][up], sfy[4][up]}
+                                      : Flux<T> {sfe[0][up - NT], sfe[1][up - NT], sfe[2][up - NT],
+                                                 sfe[3][up - NT], sfe[4][up - NT]};
+        Flux<T> acc {T(0), T(0), T(0), T(0), T(0)};
+        accumulate(acc, ds0, fx_hi, fx_lo);
+        accumulate(acc, ds1, fy_hi, fy_lo);
+        accumulate(acc, ds2, fz_hi, fz_lo);
+        if (apply) {
+            apply_update(u, dtodv, acc);
+        }
+        if (valid) {
+            st_cons(cons, plane, own, u);
+            if constexpr (Fused) {
+                Prim<T> const q1 = cons_to_prim_pt(u, gm1);
+                st_prim(prim_out, plane, own, q1);
+                T const c1 = sound_speed(gamma, q1.d, q1.p);
+                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+            }
+        }
+
+        fz_lo = fz_hi;
+        q_cur = q_next;
+        c_cur = c_next;
+        e_cur = e_next;
+    }
+    return local_min;
+}
+
+template <class T, class I, int NW, bool Fused>
+__global__ __launch_bounds__(64 * NW) void k_godunov_linear(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_plane,
+        int tiles_z,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    int const nblk = tiles_plane * tiles_z;
+    int const tb = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    T const local_min = godunov_tile_lin<T, I, NW, Fused>(prim, cons, prim_out, n0, n1, n2, kchunk, tb % tiles_plane,
+                                                           tb / tiles_plane, gamma, gm1, ds0, ds1, ds2,
+                                                           resolve_dtodv(dta), invdx0, invdx1, invdx2, true);
+    if constexpr (Fused) {
+        T const bm = block_min<T, NW>(local_min);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+        }
+    }
+}
+
 // Streamed Godunov update (2.5-D march along k) with a producer wave.
 //
 // A block owns a 64 x BY tile of interior cells in (i, j), marches along k over

@@ -59,6 +59,9 @@ const char* eb_last_error(void);
  *   variant 1: cell-parallel waves, `by` in {4, 8, 16} (default);
  *   variant 2: streamed (producer wave + global->LDS transfers), `by` in {4, 8};
  *              needs 16-byte aligned primitive fields (else EB_ERR_UNSUPPORTED);
+ *   variant 3: linear tiles of 64*by consecutive cells of a plane (rows wrap, no
+ *              lane lost to ragged x extents), `by` in {4, 8}; needs n0 <= 64*by
+ *              (else EB_ERR_UNSUPPORTED);
  *   `kchunk` >= 1 planes marched per block along k, or 0: chosen per launch from
  *   the grid (long chunks, shortened until the launch has ~512 blocks).
  *   Defaults (1, 8, 0).
@@ -259,10 +262,13 @@ typedef struct eb_batch_instance_f32 {
 int eb_batch_create_f64(eb_batch** batch, const eb_batch_instance_f64* instances, int ninstances, int index_bits);
 int eb_batch_create_f32(eb_batch** batch, const eb_batch_instance_f32* instances, int ninstances, int index_bits);
 int eb_batch_destroy(eb_batch* batch);
-/* Tile height by in {4, 8, 16}; kchunk planes per tile (0: whole columns).
- * Default (8, 32). eb_batch_autotune picks both by timing. */
-int eb_batch_set_tiles(eb_batch* batch, int by, int kchunk);
-int eb_batch_get_tiles(const eb_batch* batch, int* by, int* kchunk, int* ntiles, int* grid);
+/* Tile layout 0: 64 x by columns of cells, by in {4, 8, 16}; layout 1: runs of
+ * 64*by consecutive cells of a plane, by in {4, 8}, every instance with
+ * n0 <= 64*by. kchunk planes per tile (0: whole columns). Default (0, 8, 32).
+ * eb_batch_autotune picks all three by timing. On failure the previous layout
+ * stays in place. */
+int eb_batch_set_tiles(eb_batch* batch, int layout, int by, int kchunk);
+int eb_batch_get_tiles(const eb_batch* batch, int* layout, int* by, int* kchunk, int* ntiles, int* grid);
 /* Device arrays: dt = 3 reals per instance ([0], [1] ping-pong raw dt, [2] the
  * ghost cells' dt minimum), time = 1 double per instance. */
 int eb_batch_device_state(const eb_batch* batch, void** dt, double** time);

@@ -80,7 +80,7 @@ def test_batch_matches_oracle_loop(eb, ex, orc, dtype, index_bits):
     ens.close()
 
 
-@pytest.mark.parametrize("tiles", [(4, 0), (16, 8), (8, 1), (4, 3)])
+@pytest.mark.parametrize("tiles", [(4, 0, 0), (16, 8, 0), (8, 1, 0), (4, 3, 0), (4, 0, 1), (8, 5, 1), (4, 1, 1)])
 def test_batch_tile_layouts_are_bitwise_equal(eb, ex, orc, tiles):
     nsteps = 5
     print(f"batch tiles {tiles}", flush=True)
@@ -88,7 +88,7 @@ def test_batch_tile_layouts_are_bitwise_equal(eb, ex, orc, tiles):
     refs = _reference(orc, inst, nsteps)
     ens, P, C = _ensemble(eb, ex, inst, 32)
     ens.set_tiles(*tiles)
-    assert ens.tiles()["by"] == tiles[0]
+    assert ens.tiles()["by"] == tiles[0] and ens.tiles()["layout"] == tiles[2]
     ens.run(nsteps)
     _check(ens, P, C, refs, np.float64, nsteps, inst)
     ens.close()
@@ -135,6 +135,18 @@ def test_batch_error_paths(eb, ex, orc):
         ens.set_tiles(5, 8)
     with pytest.raises(EulerError):
         ens.set_tiles(8, -1)
+    with pytest.raises(EulerError):
+        ens.set_tiles(16, 8, layout=1)
+    ens.close()
+    # linear tiles need every row to fit a tile: a 300-wide instance refuses them
+    wide = [dict(inst[0])]
+    wide[0]["prim"] = random_prim((300, 5, 5), seed=7)
+    wide[0]["cons"] = orc.prim_to_cons(wide[0]["prim"], wide[0]["gamma"])
+    ens, _, _ = _ensemble(eb, ex, wide, 32)
+    with pytest.raises(EulerError):
+        ens.set_tiles(4, 8, layout=1)
+    assert ens.tiles()["layout"] == 0  # previous layout kept
+    ens.run(2)
     ens.close()
     tiny = [dict(inst[0])]
     tiny[0]["prim"] = np.ones((5, 2, 4, 4))
