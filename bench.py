@@ -220,6 +220,13 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
     ens.run(2, use_graph=True, per_instance=True)  # capture the per-instance pair
     ms_pi_graph = timer.run(lambda: ens.run(base_steps, use_graph=True, per_instance=True), 1) / base_steps
     ms_pi_eager = timer.run(lambda: ens.run(base_steps, per_instance=True), 1) / base_steps
+    # the single-grid kernel with linear tiles as well, so the baseline is the best per-instance form
+    cfg = eb.get_godunov_config()
+    eb.set_godunov_config(3, 8, 0)
+    ens.per_instance_graph = None
+    ens.run(2, use_graph=True, per_instance=True)
+    ms_pi_lin = timer.run(lambda: ens.run(base_steps, use_graph=True, per_instance=True), 1) / base_steps
+    eb.set_godunov_config(*cfg)
     _, t = ens.state()
     steps_done = ens.steps_done
     ens.close()
@@ -238,8 +245,11 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
         "per_instance_eager_Mcell_updates_per_s": rate(ms_pi_eager),
         "per_instance_graph_ms_per_step": round(ms_pi_graph, 4),
         "per_instance_graph_Mcell_updates_per_s": rate(ms_pi_graph),
+        "per_instance_linear_tiles_graph_ms_per_step": round(ms_pi_lin, 4),
+        "per_instance_linear_tiles_graph_Mcell_updates_per_s": rate(ms_pi_lin),
         "speedup_vs_per_instance_eager": round(ms_pi_eager / ms_graph, 3),
         "speedup_vs_per_instance_graph": round(ms_pi_graph / ms_graph, 3),
+        "speedup_vs_best_per_instance": round(min(ms_pi_eager, ms_pi_graph, ms_pi_lin) / ms_graph, 3),
         "steps_done": steps_done,
         "simulated_time_min_max": [float(t.min()), float(t.max())],
         "timed_steps": {"batched_graph": nsteps - 4, "others": base_steps},

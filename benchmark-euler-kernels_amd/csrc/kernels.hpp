@@ -12,6 +12,10 @@
 #include "device_math.hpp"
 #include "reduce.hpp"
 
+#ifndef EB_HALO_PREFETCH
+#define EB_HALO_PREFETCH 1 // tiled kernel loads its halo ring one plane ahead
+#endif
+
 namespace eb {
 
 // dt/dv for the update: either a host value, or cfl * dt_dev[0] / dv evaluated on
@@ -480,13 +484,26 @@ __device__ __forceinline__ T godunov_tile(
     }
     Prim<T> q_next {};
     Prim<T> h_cur {};
+#if EB_HALO_PREFETCH
+    if (has_halo) {
+        h_cur = ld_prim(prim, s2 * kbeg, halo);
+    }
+#endif
 
     for (I k = kbeg; k < kend; ++k) {
         I const plane = s2 * k;
         q_next = ld_prim(prim, plane + s2, own);
+#if EB_HALO_PREFETCH
+        // halo of the next plane, one plane ahead like the own cell
+        Prim<T> h_next {};
+        if (has_halo) {
+            h_next = ld_prim(prim, plane + s2, halo);
+        }
+#else
         if (has_halo) {
             h_cur = ld_prim(prim, plane, halo);
         }
+#endif
         Cons<T> u = ld_cons(cons, plane, own);
 
         // A. stage plane k.
@@ -566,6 +583,9 @@ __device__ __forceinline__ T godunov_tile(
         q_cur = q_next;
         c_cur = c_next;
         e_cur = e_next;
+#if EB_HALO_PREFETCH
+        h_cur = h_next;
+#endif
     }
     return local_min;
 }
