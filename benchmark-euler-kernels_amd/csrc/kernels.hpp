@@ -111,16 +111,16 @@ __global__ __launch_bounds__(256) void k_init_implode(PrimPtrs<T> prim, I n0, I 
     I const n = n0 * n1 * n2;
     I const stride = static_cast<I>(gridDim.x) * 256;
     for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
-    I const i = idx % n0;
-    I const j = (idx / n0) % n1;
-    I const k = idx / (n0 * n1);
-    T const c0 = ((static_cast<double>(i - 1) + 0.5) * dx0) - 0.5;
-    T const c1 = ((static_cast<double>(j - 1) + 0.5) * dx1) - 0.5;
-    T const c2 = ((static_cast<double>(k - 1) + 0.5) * dx2) - 0.5;
-    T const r2 = (c0 * c0) + (c1 * c1) + (c2 * c2);
-    T const radius2 = T(0.25 * 0.25);
-    bool const in = r2 < radius2;
-    store_prim(prim, idx, Prim<T> {in ? T(1.) : T(0.125), in ? T(1.) : T(0.14), T(0), T(0), T(0)});
+        I const i = idx % n0;
+        I const j = (idx / n0) % n1;
+        I const k = idx / (n0 * n1);
+        T const c0 = ((static_cast<double>(i - 1) + 0.5) * dx0) - 0.5;
+        T const c1 = ((static_cast<double>(j - 1) + 0.5) * dx1) - 0.5;
+        T const c2 = ((static_cast<double>(k - 1) + 0.5) * dx2) - 0.5;
+        T const r2 = (c0 * c0) + (c1 * c1) + (c2 * c2);
+        T const radius2 = T(0.25 * 0.25);
+        bool const in = r2 < radius2;
+        store_prim(prim, idx, Prim<T> {in ? T(1.) : T(0.125), in ? T(1.) : T(0.14), T(0), T(0), T(0)});
     }
 }
 
@@ -145,14 +145,14 @@ __global__ __launch_bounds__(256) void k_init_diagonal(
     I const n = n0 * n1 * n2;
     I const stride = static_cast<I>(gridDim.x) * 256;
     for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
-    I const i = idx % n0;
-    I const j = (idx / n0) % n1;
-    I const k = idx / (n0 * n1);
-    T const x0 = (static_cast<double>(i - 1) + 0.5) * dx0;
-    T const x1 = (static_cast<double>(j - 1) + 0.5) * dx1;
-    T const x2 = (static_cast<double>(k - 1) + 0.5) * dx2;
-    bool const hi = (x0 + x1 + x2) > offset;
-    store_prim(prim, idx, Prim<T> {hi ? d_hi : d_lo, hi ? p_hi : p_lo, T(0), T(0), T(0)});
+        I const i = idx % n0;
+        I const j = (idx / n0) % n1;
+        I const k = idx / (n0 * n1);
+        T const x0 = (static_cast<double>(i - 1) + 0.5) * dx0;
+        T const x1 = (static_cast<double>(j - 1) + 0.5) * dx1;
+        T const x2 = (static_cast<double>(k - 1) + 0.5) * dx2;
+        bool const hi = (x0 + x1 + x2) > offset;
+        store_prim(prim, idx, Prim<T> {hi ? d_hi : d_lo, hi ? p_hi : p_lo, T(0), T(0), T(0)});
     }
 }
 
@@ -187,15 +187,15 @@ __global__ __launch_bounds__(256) void k_init_random(
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
     for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
-    unsigned long long const g = static_cast<unsigned long long>(idx);
-    double const d = d_lo + (d_hi - d_lo) * uniform01(seed, 0, g);
-    double const p = p_lo + (p_hi - p_lo) * uniform01(seed, 1, g);
-    double u[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double const a = 1.0 - uniform01(seed, 2 + 2 * c, g);
-        double const b = uniform01(seed, 3 + 2 * c, g);
-        u[c] = u_sigma * sqrt(-2.0 * log(a)) * cos(6.283185307179586 * b);
+        unsigned long long const g = static_cast<unsigned long long>(idx);
+        double const d = d_lo + (d_hi - d_lo) * uniform01(seed, 0, g);
+        double const p = p_lo + (p_hi - p_lo) * uniform01(seed, 1, g);
+        double u[3];
+    #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double const a = 1.0 - uniform01(seed, 2 + 2 * c, g);
+            double const b = uniform01(seed, 3 + 2 * c, g);
+            u[c] = u_sigma * sqrt(-2.0 * log(a)) * cos(6.283185307179586 * b);
     }
     store_prim(prim, idx, Prim<T> {T(d), T(p), T(u[0]), T(u[1]), T(u[2])});
     }
