@@ -67,21 +67,18 @@ __device__ __forceinline__ float fast_rcp(float x)
     return 1.0f / x;
 }
 
-// Per-cell quantities the face derivatives share: the sound speed c and the
-// factors 1/(2p), 1/(2rho) of its derivatives (dc = c (dp/(2p) - drho/(2rho))).
+// Per-cell quantity the face derivatives share: the sound speed c.
 template <class T>
 struct VjpCell
 {
     Prim<T> q;
     T c;
-    T r2p;
-    T r2d;
 };
 
 template <class T>
 __device__ __forceinline__ VjpCell<T> vjp_cell(Prim<T> const& q, T gamma)
 {
-    return {q, sound_speed(gamma, q.d, q.p), T(0.5) * fast_rcp(q.p), T(0.5) * fast_rcp(q.d)};
+    return {q, sound_speed(gamma, q.d, q.p)};
 }
 
 // Adds (dF/dqL)^T g to gl and (dF/dqR)^T g to gr. Forward values are recomputed
@@ -266,20 +263,20 @@ __device__ __forceinline__ void hllc_vjp(
     } else {
         g_unL += g_SR;
     }
-    T g_cL = T(0);
-    T g_cR = T(0);
-    if (sel_c) {
-        g_cR = g_cmax;
-    } else {
-        g_cL = g_cmax;
+    // c = sqrt(gamma*p/d): dc = c/2 * (dp/p - dd/d), on the side that set cmax
+    {
+        Prim<T> const& qc = sel_c ? qR : qL;
+        T const gcc = g_cmax * cmax;
+        T const gp = gcc * (T(0.5) * fast_rcp(qc.p));
+        T const gd = gcc * (T(0.5) * fast_rcp(qc.d));
+        if (sel_c) {
+            g_pR += gp;
+            g_dR -= gd;
+        } else {
+            g_pL += gp;
+            g_dL -= gd;
+        }
     }
-    // c = sqrt(gamma*p/d): dc = c/2 * (dp/p - dd/d)
-    T const gcL = g_cL * cL;
-    T const gcR = g_cR * cR;
-    g_pL += gcL * L.r2p;
-    g_dL -= gcL * L.r2d;
-    g_pR += gcR * R.r2p;
-    g_dR -= gcR * R.r2d;
     gl.d += g_dL;
     gl.p += g_pL;
     comp<Dir>(gl) += g_unL;

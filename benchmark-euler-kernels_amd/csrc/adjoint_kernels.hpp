@@ -23,7 +23,7 @@
 #include "kernels.hpp"
 
 #ifndef EB_VJP_MIN_WAVES
-#define EB_VJP_MIN_WAVES 1 // waves per SIMD the tiled adjoint kernel is compiled for (1: no spills)
+#define EB_VJP_MIN_WAVES 2 // waves per SIMD the tiled adjoint kernel is compiled for
 #endif
 
 namespace eb {
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(256) void k_godunov_vjp_ghost(
 
 // Tiled, face-once adjoint step over the interior cells (the forward kernel's
 // geometry: 64 x BY tiles of interior cells, a march along k over `kchunk` planes).
-// Per plane k each thread stages its cell (primitives, c, 1/(2p), 1/(2rho) and the
-// incoming adjoint mu) in LDS and differentiates the x and y faces on its low side
+// Per plane k each thread stages its cell (primitives, c and the incoming adjoint
+// mu) in LDS and differentiates the x and y faces on its low side
 // and the z face on its high side; two of the waves also take the tile's high-side
 // x and y faces. Each face's two halves go to its two cells: the low cell's half
 // by wave shuffle (x), LDS (y) or registers carried to the next plane (z). Faces
@@ -262,10 +262,11 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
     constexpr int LX = BX + 2;
     constexpr int LY = BY + 2;
     constexpr int NHALO = 2 * BX + 2 * BY;
-    constexpr int NF = 13; // d, p, u0, u1, u2, c, 1/(2p), 1/(2rho), mu (5)
+    constexpr int NF = 11; // d, p, u0, u1, u2, c, mu (5)
     __shared__ T sq[NF][LY][LX];
     __shared__ T sgy[5][BY + 1][BX]; // low-cell halves of the y faces: row r = face between tile rows r-1, r
     __shared__ T sgx[5][BY];         // low-cell halves of the tile's high-side x faces
+    __shared__ T sgl[5][BY][BX];     // low-cell halves of the x faces: [r][c] belongs to cell c-1 of row r
 
     int const nblk = tiles_x * tiles_y * tiles_z;
     int const t = xcd_tile(static_cast<int>(blockIdx.x) - ghost_blocks, nblk);
@@ -330,19 +331,16 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         sq[3][r][c] = v.q.u1;
         sq[4][r][c] = v.q.u2;
         sq[5][r][c] = v.c;
-        sq[6][r][c] = v.r2p;
-        sq[7][r][c] = v.r2d;
-        sq[8][r][c] = w.d;
-        sq[9][r][c] = w.e;
-        sq[10][r][c] = w.m0;
-        sq[11][r][c] = w.m1;
-        sq[12][r][c] = w.m2;
+        sq[6][r][c] = w.d;
+        sq[7][r][c] = w.e;
+        sq[8][r][c] = w.m0;
+        sq[9][r][c] = w.m1;
+        sq[10][r][c] = w.m2;
     };
     auto cell = [&](int r, int c) {
-        return VjpCell<T> {Prim<T> {sq[0][r][c], sq[1][r][c], sq[2][r][c], sq[3][r][c], sq[4][r][c]}, sq[5][r][c],
-                           sq[6][r][c], sq[7][r][c]};
+        return VjpCell<T> {Prim<T> {sq[0][r][c], sq[1][r][c], sq[2][r][c], sq[3][r][c], sq[4][r][c]}, sq[5][r][c]};
     };
-    auto weight = [&](int r, int c) { return Flux<T> {sq[8][r][c], sq[9][r][c], sq[10][r][c], sq[11][r][c], sq[12][r][c]}; };
+    auto weight = [&](int r, int c) { return Flux<T> {sq[6][r][c], sq[7][r][c], sq[8][r][c], sq[9][r][c], sq[10][r][c]}; };
     PrimGrad<T> const zero {T(0), T(0), T(0), T(0), T(0)};
 
     // Prologue: the z face below the first plane gives the own cell's first part.
@@ -379,8 +377,8 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         }
         if (__syncthreads_or(active ? 1 : 0) == 0) {
             if (own_int) {
-                vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), v_cur.q, w_cur, carry,
-                                       dtodv, gm1);
+                vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), cell(ty + 1, tx + 1).q,
+                                       weight(ty + 1, tx + 1), carry, dtodv, gm1);
             }
             carry = zero;
             v_cur = vjp_cell(q_next, gamma);
@@ -388,17 +386,27 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
             continue;
         }
 
-        // B. face derivatives.
-        PrimGrad<T> gl_x = zero;
+        // B. face derivatives. The own cell is read back from LDS where it is needed,
+        // so that no copy of it stays in registers across the derivatives.
         PrimGrad<T> gq = carry;
-        hllc_vjp<0>(cell(ty + 1, tx), v_cur, gm1,
-                    face_weight(ds0, weight(ty + 1, tx), col_int(i - 1) && row_ok, w_cur, own_int), gl_x, gq);
+        {
+            PrimGrad<T> gl_x = zero;
+            hllc_vjp<0>(cell(ty + 1, tx), cell(ty + 1, tx + 1), gm1,
+                        face_weight(ds0, weight(ty + 1, tx), col_int(i - 1) && row_ok, weight(ty + 1, tx + 1),
+                                    own_int),
+                        gl_x, gq);
+            sgl[0][ty][tx] = gl_x.d;
+            sgl[1][ty][tx] = gl_x.p;
+            sgl[2][ty][tx] = gl_x.u0;
+            sgl[3][ty][tx] = gl_x.u1;
+            sgl[4][ty][tx] = gl_x.u2;
+        }
         sched_fence();
         {
             PrimGrad<T> gl_y = zero;
-            hllc_vjp<1>(cell(ty, tx + 1), v_cur, gm1,
-                        face_weight(ds1, weight(ty, tx + 1), col_int(i) && j - 1 >= 1 && j - 1 <= n1 - 2, w_cur,
-                                    own_int),
+            hllc_vjp<1>(cell(ty, tx + 1), cell(ty + 1, tx + 1), gm1,
+                        face_weight(ds1, weight(ty, tx + 1), col_int(i) && j - 1 >= 1 && j - 1 <= n1 - 2,
+                                    weight(ty + 1, tx + 1), own_int),
                         gl_y, gq);
             sgy[0][ty][tx] = gl_y.d;
             sgy[1][ty][tx] = gl_y.p;
@@ -411,9 +419,10 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
             // High-side y faces of the tile: own row and the halo row above it.
             PrimGrad<T> gl = zero;
             PrimGrad<T> dummy = zero;
-            hllc_vjp<1>(v_cur, cell(BY + 1, tx + 1), gm1,
-                        face_weight(ds1, w_cur, own_int, weight(BY + 1, tx + 1), col_int(i) && j + 1 <= n1 - 2), gl,
-                        dummy);
+            hllc_vjp<1>(cell(BY, tx + 1), cell(BY + 1, tx + 1), gm1,
+                        face_weight(ds1, weight(BY, tx + 1), own_int, weight(BY + 1, tx + 1),
+                                    col_int(i) && j + 1 <= n1 - 2),
+                        gl, dummy);
             sgy[0][BY][tx] = gl.d;
             sgy[1][BY][tx] = gl.p;
             sgy[2][BY][tx] = gl.u0;
@@ -440,26 +449,21 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         sched_fence();
         VjpCell<T> const v_next = vjp_cell(q_next, gamma);
         PrimGrad<T> gr_z = zero;
-        hllc_vjp<2>(v_cur, v_next, gm1, face_weight(ds2, w_cur, own_int, w_next, own_int && k + 1 <= n2 - 2), gq,
-                    gr_z);
+        hllc_vjp<2>(cell(ty + 1, tx + 1), v_next, gm1,
+                    face_weight(ds2, weight(ty + 1, tx + 1), own_int, w_next, own_int && k + 1 <= n2 - 2), gq, gr_z);
         lds_barrier();
 
         // C. the low-cell halves from the next lane (x) and the next row (y).
-        PrimGrad<T> hx;
-        hx.d = __shfl_down(gl_x.d, 1, 64);
-        hx.p = __shfl_down(gl_x.p, 1, 64);
-        hx.u0 = __shfl_down(gl_x.u0, 1, 64);
-        hx.u1 = __shfl_down(gl_x.u1, 1, 64);
-        hx.u2 = __shfl_down(gl_x.u2, 1, 64);
-        if (tx == BX - 1) {
-            hx = PrimGrad<T> {sgx[0][ty], sgx[1][ty], sgx[2][ty], sgx[3][ty], sgx[4][ty]};
-        }
+        PrimGrad<T> const hx = tx < BX - 1
+                                       ? PrimGrad<T> {sgl[0][ty][tx + 1], sgl[1][ty][tx + 1], sgl[2][ty][tx + 1],
+                                                      sgl[3][ty][tx + 1], sgl[4][ty][tx + 1]}
+                                       : PrimGrad<T> {sgx[0][ty], sgx[1][ty], sgx[2][ty], sgx[3][ty], sgx[4][ty]};
         add_to(gq, hx);
         add_to(gq, PrimGrad<T> {sgy[0][ty + 1][tx], sgy[1][ty + 1][tx], sgy[2][ty + 1][tx], sgy[3][ty + 1][tx],
                                 sgy[4][ty + 1][tx]});
         if (own_int) {
-            vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), v_cur.q, w_cur, gq, dtodv,
-                                   gm1);
+            vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), cell(ty + 1, tx + 1).q,
+                                   weight(ty + 1, tx + 1), gq, dtodv, gm1);
         }
 
         carry = gr_z;
