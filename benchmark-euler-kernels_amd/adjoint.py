@@ -119,8 +119,10 @@ class Rollout:
     # --------------------------------------------------------------- backward
     def _recompute(self, start: int, stop: int) -> List[ops.EulerPrimArrays]:
         """P_n for n in [start, stop), recomputed from the checkpoint at `start`
-        with the recorded time steps (bitwise the forward's states)."""
-        cons = _clone_cons(self.checkpoints[start])
+        with the recorded time steps (bitwise the forward's states). The backward
+        visits each segment once, so the checkpoint itself becomes the working state
+        (and its memory is released with the segment)."""
+        cons = self.checkpoints.pop(start)
         need = stop - start
         while len(self._segment) < need:
             b = _empty_prim_like(cons)
@@ -138,9 +140,12 @@ class Rollout:
         return prims
 
     def backward(self, gprim_final: Sequence[torch.Tensor]) -> ops.EulerPrimArrays:
-        """dL/dP_0 from dL/dP_N (five tensors in the primitive field order)."""
+        """dL/dP_0 from dL/dP_N (five tensors in the primitive field order). Consumes
+        the checkpoints: one backward per forward."""
         if self.prim_final is None:
             raise RuntimeError("backward before forward")
+        if len(self.checkpoints) != (self.nsteps + self.k - 1) // self.k:
+            raise RuntimeError("the checkpoints of this rollout were already consumed by a backward pass")
         suf = self.suf
         r = lambda x: ops._real(suf, x)  # noqa: E731
         geom = self._geom()
@@ -154,7 +159,7 @@ class Rollout:
              ctypes.byref(ops.prim_ptrs(gP)), ctypes.byref(ops.cons_ptrs(mu)), n0, n1, n2, bits,
              r(self.eos.gamma))
         gprim0 = _empty_prim_like(self.prim0)
-        starts = sorted(self.checkpoints)
+        starts = sorted(self.checkpoints)  # consumed segment by segment below
         for start in reversed(starts):
             stop = min(start + self.k, self.nsteps)
             prims = self._recompute(start, stop)
