@@ -9,8 +9,9 @@ c2p(U_n), exactly as the reference loop does.
 Backward: conservative states are checkpointed every `checkpoint_every` steps;
 each segment is recomputed from its checkpoint (with the recorded dt, so it is
 bitwise the forward) and then reversed step by step through the hand-written HIP
-vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). Memory: one state
-per checkpoint plus one primitive state per step of a segment.
+vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). The last segment's
+primitive states are kept by the forward pass itself. Memory: one state per
+checkpoint plus one primitive state per step of a segment.
 """
 from __future__ import annotations
 
@@ -62,6 +63,8 @@ class Rollout:
         self.prim_final: Optional[ops.EulerPrimArrays] = None
         self._slots: Optional[torch.Tensor] = None  # [0]: discarded dt, [1]: ghost-cell dt minimum
         self._segment: List[ops.EulerPrimArrays] = []
+        self.tail = 0
+        self._tail_ready = False
 
     # ---------------------------------------------------------------- launches
     def _geom(self):
@@ -102,19 +105,37 @@ class Rollout:
             dt = torch.empty(self.nsteps + 1, dtype=dtype, device=dev)
             ops.time_step_async(self.ex, self.prim0, self.eos, self.mesh, out=dt[0:1])
         self.dts = dt[:self.nsteps]
+        # The last segment's states are written straight into the segment buffers,
+        # so the backward starts without recomputing it (and needs no checkpoint for it).
+        self.tail = ((self.nsteps - 1) // self.k) * self.k
+        self._ensure_segment(cons, self.nsteps - self.tail)
         buf = [_clone_prim(prim0), _empty_prim_like(prim0)]
         self._ghost_refresh(cons, buf[1])
         self.checkpoints = {}
+        cur = buf[0]
         for n in range(self.nsteps):
-            if n % self.k == 0:
+            if n % self.k == 0 and n != self.tail:
                 self.checkpoints[n] = _clone_cons(cons)
+            m = n + 1 - self.tail
+            if 0 <= m < self.nsteps - self.tail:
+                nxt = self._segment[m]
+            else:
+                nxt = buf[1] if cur is buf[0] else buf[0]
             out = self._slots[0:].data_ptr() if frozen_dts is not None else dt[n + 1:].data_ptr()
-            self._fused(buf[n % 2], cons, buf[1 - n % 2], dt[n:].data_ptr(), out)
+            self._fused(cur, cons, nxt, dt[n:].data_ptr(), out)
             if n == 0:
                 # later steps read ghost primitives round-tripped through cons_to_prim
                 self._ghost_refresh(cons, buf[0])
-        self.prim_final = buf[self.nsteps % 2]
+            cur = nxt
+        self.prim_final = cur
+        self._tail_ready = True
         return _clone_prim(self.prim_final)
+
+    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int):
+        while len(self._segment) < need:
+            b = _empty_prim_like(cons)
+            self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
+            self._segment.append(b)
 
     # --------------------------------------------------------------- backward
     def _recompute(self, start: int, stop: int) -> List[ops.EulerPrimArrays]:
@@ -123,11 +144,7 @@ class Rollout:
         visits each segment once, so the checkpoint itself becomes the working state
         (and its memory is released with the segment)."""
         cons = self.checkpoints.pop(start)
-        need = stop - start
-        while len(self._segment) < need:
-            b = _empty_prim_like(cons)
-            self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
-            self._segment.append(b)
+        self._ensure_segment(cons, stop - start)
         if start == 0:
             prims = [self.prim0]
         else:
@@ -144,8 +161,9 @@ class Rollout:
         the checkpoints: one backward per forward."""
         if self.prim_final is None:
             raise RuntimeError("backward before forward")
-        if len(self.checkpoints) != (self.nsteps + self.k - 1) // self.k:
+        if not self._tail_ready or len(self.checkpoints) != self.tail // self.k:
             raise RuntimeError("the checkpoints of this rollout were already consumed by a backward pass")
+        self._tail_ready = False
         suf = self.suf
         r = lambda x: ops._real(suf, x)  # noqa: E731
         geom = self._geom()
@@ -159,10 +177,15 @@ class Rollout:
              ctypes.byref(ops.prim_ptrs(gP)), ctypes.byref(ops.cons_ptrs(mu)), n0, n1, n2, bits,
              r(self.eos.gamma))
         gprim0 = _empty_prim_like(self.prim0)
-        starts = sorted(self.checkpoints)  # consumed segment by segment below
+        starts = sorted(self.checkpoints) + [self.tail]  # consumed segment by segment below
         for start in reversed(starts):
             stop = min(start + self.k, self.nsteps)
-            prims = self._recompute(start, stop)
+            if start == self.tail:
+                # stored by the forward pass
+                head = self.prim0 if start == 0 else self._segment[0]
+                prims = [head] + self._segment[1:stop - start]
+            else:
+                prims = self._recompute(start, stop)
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
                 dt = self.dts[n:n + 1].data_ptr()
