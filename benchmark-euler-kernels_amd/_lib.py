@@ -102,9 +102,11 @@ def _signatures():
         sig[f"eb_init_implode_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 3
         sig[f"eb_init_diagonal_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 8
         sig[f"eb_init_random_{suf}"] = [_vp, _P(PrimPtrs), _i64, _int, _u64] + [_f64] * 5
-        sig[f"eb_godunov_vjp_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(ConsPtrs)] + ext + [real] * 4 + [_vp, real]
+        sig[f"eb_godunov_vjp_{suf}"] = (
+            [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(ConsPtrs)] + ext + [real] * 4 + [_vp, real, _vp]
+        )
         sig[f"eb_godunov_vjp_first_{suf}"] = (
-            [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 4 + [_vp, real]
+            [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 4 + [_vp, real, _vp]
         )
         sig[f"eb_cons_to_prim_vjp_{suf}"] = [_vp, _P(PrimPtrs), _P(PrimPtrs), _P(ConsPtrs)] + ext + [real]
         sig[f"eb_prim_to_cons_vjp_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real]

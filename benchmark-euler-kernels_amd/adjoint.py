@@ -40,6 +40,14 @@ def _empty_prim_like(a: ops.Arrays) -> ops.EulerPrimArrays:
     return ops.EulerPrimArrays(*(torch.empty_like(t) for t in a.tensors()), index_bits=a.index_bits)
 
 
+def _zeros_cons_like(a: ops.Arrays) -> ops.EulerConsArrays:
+    return ops.EulerConsArrays(*(torch.zeros_like(t) for t in a.tensors()), index_bits=a.index_bits)
+
+
+def _zeros_prim_like(a: ops.Arrays) -> ops.EulerPrimArrays:
+    return ops.EulerPrimArrays(*(torch.zeros_like(t) for t in a.tensors()), index_bits=a.index_bits)
+
+
 class Rollout:
     """N-step rollout with checkpoints and its adjoint (dt frozen per step).
 
@@ -171,12 +179,29 @@ class Rollout:
         h = self.ex.handle
 
         gP = ops.EulerPrimArrays(*(g.contiguous() for g in gprim_final), index_bits=bits)
-        mu = _empty_cons_like(self.prim0)
-        mu_next = _empty_cons_like(self.prim0)
+        # The adjoint spreads by at most one cell per step from the support of the
+        # incoming gradient: each step only processes that box (outputs elsewhere stay
+        # exactly zero, hence the zero-filled buffers).
+        nz = torch.stack([g != 0 for g in gP.tensors()]).any(dim=0)
+        idx = nz[1:-1, 1:-1, 1:-1].nonzero()
+        box = None
+        if idx.numel() > 0:
+            lo = (idx.min(dim=0).values + 1).tolist()  # (k, j, i), array coordinates
+            hi = (idx.max(dim=0).values + 1).tolist()
+            box = [lo[2], hi[2], lo[1], hi[1], lo[0], hi[0]]
+        mu = _zeros_cons_like(self.prim0)
+        mu_next = _zeros_cons_like(self.prim0)
         call(f"eb_cons_to_prim_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(self.prim_final)),
              ctypes.byref(ops.prim_ptrs(gP)), ctypes.byref(ops.cons_ptrs(mu)), n0, n1, n2, bits,
              r(self.eos.gamma))
-        gprim0 = _empty_prim_like(self.prim0)
+        gprim0 = _zeros_prim_like(self.prim0)
+        step = 0
+
+        def region():
+            if box is None:
+                return (ctypes.c_int64 * 6)(1, 0, 1, 0, 1, 0)  # empty: ghost cells only
+            g = step
+            return (ctypes.c_int64 * 6)(box[0] - g, box[1] + g, box[2] - g, box[3] + g, box[4] - g, box[5] + g)
         starts = sorted(self.checkpoints) + [self.tail]  # consumed segment by segment below
         for start in reversed(starts):
             stop = min(start + self.k, self.nsteps)
@@ -189,14 +214,15 @@ class Rollout:
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
                 dt = self.dts[n:n + 1].data_ptr()
+                step += 1
                 if n == 0:
                     call(f"eb_godunov_vjp_first_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
                          ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.prim_ptrs(gprim0)), *geom, dt,
-                         r(self.cfl))
+                         r(self.cfl), region())
                 else:
                     call(f"eb_godunov_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
                          ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.cons_ptrs(mu_next)), *geom, dt,
-                         r(self.cfl))
+                         r(self.cfl), region())
                     mu, mu_next = mu_next, mu
         return gprim0
 

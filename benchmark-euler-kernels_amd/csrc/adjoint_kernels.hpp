@@ -158,40 +158,38 @@ __device__ __forceinline__ void vjp_ghost_cells(
         bool const ii = inside(i, n0);
         bool const jj = inside(j, n1);
         bool const kk = inside(k, n2);
-        VjpCell<T> const vg = vjp_cell(q, gamma);
-        auto nb = [&](I idx) {
-            Cons<T> const v = load_cons(mu, idx);
-            return Flux<T> {v.d, v.e, v.m0, v.m1, v.m2};
-        };
+        // the one interior neighbour, if any, and which side of the face the ghost is on
+        int dir = -1;
+        bool ghost_low = false;
+        I other = c;
         if (jj && kk && (i == 0 || i == n0 - 1) && n0 >= 3) {
-            if (i == 0) {
-                I const b = c + 1;
-                hllc_vjp<0>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds0, w, false, nb(b), true),
-                            gq, dummy);
-            } else {
-                I const a = c - 1;
-                hllc_vjp<0>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds0, nb(a), true, w, false),
-                            dummy, gq);
-            }
+            dir = 0;
+            ghost_low = i == 0;
+            other = ghost_low ? c + 1 : c - 1;
         } else if (ii && kk && (j == 0 || j == n1 - 1) && n1 >= 3) {
-            if (j == 0) {
-                I const b = c + n0;
-                hllc_vjp<1>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds1, w, false, nb(b), true),
-                            gq, dummy);
-            } else {
-                I const a = c - n0;
-                hllc_vjp<1>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds1, nb(a), true, w, false),
-                            dummy, gq);
-            }
+            dir = 1;
+            ghost_low = j == 0;
+            other = ghost_low ? c + n0 : c - n0;
         } else if (ii && jj && (k == 0 || k == n2 - 1) && n2 >= 3) {
-            if (k == 0) {
-                I const b = c + n0 * n1;
-                hllc_vjp<2>(vg, vjp_cell(load_prim(prim, b), gamma), gm1, face_weight(ds2, w, false, nb(b), true),
-                            gq, dummy);
-            } else {
-                I const a = c - n0 * n1;
-                hllc_vjp<2>(vjp_cell(load_prim(prim, a), gamma), vg, gm1, face_weight(ds2, nb(a), true, w, false),
-                            dummy, gq);
+            dir = 2;
+            ghost_low = k == 0;
+            other = ghost_low ? c + n0 * n1 : c - n0 * n1;
+        }
+        if (dir >= 0) {
+            Cons<T> const uo = load_cons(mu, other);
+            Flux<T> const wo {uo.d, uo.e, uo.m0, uo.m1, uo.m2};
+            T const ds = dir == 0 ? ds0 : (dir == 1 ? ds1 : ds2);
+            Flux<T> const g = ghost_low ? face_weight(ds, w, false, wo, true) : face_weight(ds, wo, true, w, false);
+            if (nonzero(g)) { // the face carries no adjoint otherwise
+                VjpCell<T> const vg = vjp_cell(q, gamma);
+                VjpCell<T> const vo = vjp_cell(load_prim(prim, other), gamma);
+                if (dir == 0) {
+                    ghost_low ? hllc_vjp<0>(vg, vo, gm1, g, gq, dummy) : hllc_vjp<0>(vo, vg, gm1, g, dummy, gq);
+                } else if (dir == 1) {
+                    ghost_low ? hllc_vjp<1>(vg, vo, gm1, g, gq, dummy) : hllc_vjp<1>(vo, vg, gm1, g, dummy, gq);
+                } else {
+                    ghost_low ? hllc_vjp<2>(vg, vo, gm1, g, gq, dummy) : hllc_vjp<2>(vo, vg, gm1, g, dummy, gq);
+                }
             }
         }
         vjp_output<T, I, Mode>(mu_out, gprim, c, q, w, gq, dtodv, gm1);
@@ -239,6 +237,9 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         I n1,
         I n2,
         int kchunk,
+        int tile0_x,
+        int tile0_y,
+        int tile0_z,
         int tiles_x,
         int tiles_y,
         int tiles_z,
@@ -270,9 +271,9 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
 
     int const nblk = tiles_x * tiles_y * tiles_z;
     int const t = xcd_tile(static_cast<int>(blockIdx.x) - ghost_blocks, nblk);
-    int const bx = t % tiles_x;
-    int const by = (t / tiles_x) % tiles_y;
-    int const bz = t / (tiles_x * tiles_y);
+    int const bx = tile0_x + t % tiles_x;
+    int const by = tile0_y + (t / tiles_x) % tiles_y;
+    int const bz = tile0_z + t / (tiles_x * tiles_y);
     int const tx = threadIdx.x;
     int const ty = threadIdx.y;
     int const lid = tx + BX * ty;

@@ -765,7 +765,8 @@ int abi_init_random(hipStream_t s, PS const* prim, int64_t n, int bits, uint64_t
 
 template <class T, class PS, class CS>
 int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu_out, PS const* gprim, int64_t n0,
-                    int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T const* dt_dev, T cfl)
+                    int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T const* dt_dev, T cfl,
+                    int64_t const* region)
 {
     EB_CHECK(check_extents(n0, n1, n2, bits));
     bool const first = gprim != nullptr;
@@ -798,25 +799,51 @@ int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu
 #endif
     constexpr int kBy = EB_VJP_BY;
     bool const interior = n0 >= 3 && n1 >= 3 && n2 >= 3;
+    // Interior cells [lo, hi] per axis to process (all of them by default): cells
+    // outside keep their mu_out / gprim untouched, which is exact when mu_in and
+    // the result vanish there (a localised adjoint).
+    int64_t lo[3] = {1, 1, 1};
+    int64_t hi[3] = {n0 - 2, n1 - 2, n2 - 2};
+    if (region != nullptr) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = region[2 * d] > lo[d] ? region[2 * d] : lo[d];
+            hi[d] = region[2 * d + 1] < hi[d] ? region[2 * d + 1] : hi[d];
+        }
+    }
+    bool const tiles = interior && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
     int tiles_x = 0;
     int tiles_y = 0;
     int tiles_z = 0;
+    int tile0_x = 0;
+    int tile0_y = 0;
+    int tile0_z = 0;
     int kchunk = 1;
     if (interior) {
         if (static_cast<__int128>(n0) * n1 * static_cast<int64_t>(sizeof(T)) >= (static_cast<__int128>(1) << 32)) {
             return fail(EB_ERR_UNSUPPORTED, "a k-plane must hold fewer than 2^32 bytes per field");
         }
-        int64_t const columns = ((n0 - 2 + 63) / 64) * ((n1 - 2 + kBy - 1) / kBy);
-        int64_t kc = (n2 - 2) * columns / 1024; // about 1024 blocks
+    }
+    if (tiles) {
+        int64_t const x0 = (lo[0] - 1) / 64;
+        int64_t const x1 = (hi[0] - 1) / 64 + 1;
+        int64_t const y0 = (lo[1] - 1) / kBy;
+        int64_t const y1 = (hi[1] - 1) / kBy + 1;
+        int64_t const columns = (x1 - x0) * (y1 - y0);
+        int64_t const nz = hi[2] - lo[2] + 1;
+        int64_t kc = nz * columns / 1024; // about 1024 blocks
         kc = kc < 4 ? 4 : (kc > 64 ? 64 : kc);
         kchunk = static_cast<int>(kc < n2 - 2 ? kc : n2 - 2);
-        int64_t const tz = (n2 - 2 + kchunk - 1) / kchunk;
-        if (columns * tz > std::numeric_limits<int32_t>::max()) {
+        int64_t const z0 = (lo[2] - 1) / kchunk;
+        int64_t const z1 = (hi[2] - 1) / kchunk + 1;
+        if (columns * (z1 - z0) > std::numeric_limits<int32_t>::max() / 2) {
             return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
         }
-        tiles_x = static_cast<int>((n0 - 2 + 63) / 64);
-        tiles_y = static_cast<int>((n1 - 2 + kBy - 1) / kBy);
-        tiles_z = static_cast<int>(tz);
+        tile0_x = static_cast<int>(x0);
+        tile0_y = static_cast<int>(y0);
+        tile0_z = static_cast<int>(z0);
+        tiles_x = static_cast<int>(x1 - x0);
+        tiles_y = static_cast<int>(y1 - y0);
+        tiles_z = static_cast<int>(z1 - z0);
     }
     int64_t const inner = interior ? (n0 - 2) * (n1 - 2) * (n2 - 2) : 0;
     // Ghost cells ride in the first blocks of the tiled launch (a multiple of 8,
@@ -830,8 +857,8 @@ int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu
     do {                                                                                                             \
         if (interior) {                                                                                              \
             hipLaunchKernelGGL((eb::k_godunov_vjp_tiled<T, I, kBy, M>), tgrid, dim3(64, kBy), 0, s, pc, mic, mo, gp, \
-                               static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y, \
-                               tiles_z, ghost_blocks, gamma, gm1, ds0, ds1, ds2, dta);                               \
+                               static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tile0_x, tile0_y, \
+                               tile0_z, tiles_x, tiles_y, tiles_z, ghost_blocks, gamma, gm1, ds0, ds1, ds2, dta);    \
         } else {                                                                                                     \
             hipLaunchKernelGGL((eb::k_godunov_vjp_ghost<T, I, M>), ggrid, dim3(256), 0, s, pc, mic, mo, gp,           \
                                static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma, gm1, ds0, ds1, ds2, \
@@ -1063,17 +1090,17 @@ extern "C" {
     }                                                                                                                \
     int eb_godunov_vjp_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* mu_in,                    \
                              const eb_cons_##SUF* mu_out, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma,     \
-                             T dx0, T dx1, T dx2, const T* dt_dev, T cfl)                                            \
+                             T dx0, T dx1, T dx2, const T* dt_dev, T cfl, const int64_t* region)                     \
     {                                                                                                                \
         return abi_godunov_vjp<T, eb_prim_##SUF, eb_cons_##SUF>(s, p, mu_in, mu_out, nullptr, n0, n1, n2, bits,     \
-                                                                gamma, dx0, dx1, dx2, dt_dev, cfl);                   \
+                                                                gamma, dx0, dx1, dx2, dt_dev, cfl, region);           \
     }                                                                                                                \
     int eb_godunov_vjp_first_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_cons_##SUF* mu_in,              \
                                    const eb_prim_##SUF* gprim, int64_t n0, int64_t n1, int64_t n2, int bits,        \
-                                   T gamma, T dx0, T dx1, T dx2, const T* dt_dev, T cfl)                             \
+                                   T gamma, T dx0, T dx1, T dx2, const T* dt_dev, T cfl, const int64_t* region)      \
     {                                                                                                                \
         return abi_godunov_vjp<T, eb_prim_##SUF, eb_cons_##SUF>(s, p, mu_in, nullptr, gprim, n0, n1, n2, bits,      \
-                                                                gamma, dx0, dx1, dx2, dt_dev, cfl);                   \
+                                                                gamma, dx0, dx1, dx2, dt_dev, cfl, region);           \
     }                                                                                                                \
     int eb_cons_to_prim_vjp_##SUF(hipStream_t s, const eb_prim_##SUF* p, const eb_prim_##SUF* gp,                 \
                                   const eb_cons_##SUF* gc, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma)  \

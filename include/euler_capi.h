@@ -175,19 +175,28 @@ int eb_init_random_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n, i
  * mu_* are adjoints of the conservatives (cons layout), all cells.
  *   eb_godunov_vjp:       mu_out = mu_in + c2p^T(-dtodv * R^T(P) mu_in|interior)
  *   eb_godunov_vjp_first: gprim  = -dtodv * R^T(P_0) mu_in|interior + p2c^T mu_in
- * (the first step reads the given P_0, later steps read c2p(U_n)). */
+ * (the first step reads the given P_0, later steps read c2p(U_n)).
+ * region (nullable): interior cells {i_lo, i_hi, j_lo, j_hi, k_lo, k_hi}
+ * (inclusive) whose outputs are computed; interior cells outside it are not
+ * written (ghost cells always are). Exact for a localised adjoint: when mu_in
+ * vanishes outside the region shrunk by one cell, so does the result outside the
+ * region, and zero-filled outputs stay correct. */
 int eb_godunov_vjp_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* mu_in,
                        const eb_cons_f64* mu_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
-                       double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl);
+                       double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl,
+                       const int64_t* region);
 int eb_godunov_vjp_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* mu_in,
                        const eb_cons_f32* mu_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
-                       float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl);
+                       float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl,
+                       const int64_t* region);
 int eb_godunov_vjp_first_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_cons_f64* mu_in,
                              const eb_prim_f64* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
-                             double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl);
+                             double gamma, double dx0, double dx1, double dx2, const double* dt_dev, double cfl,
+                             const int64_t* region);
 int eb_godunov_vjp_first_f32(hipStream_t stream, const eb_prim_f32* prim, const eb_cons_f32* mu_in,
                              const eb_prim_f32* gprim, int64_t n0, int64_t n1, int64_t n2, int index_bits,
-                             float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl);
+                             float gamma, float dx0, float dx1, float dx2, const float* dt_dev, float cfl,
+                             const int64_t* region);
 /* gcons = (d prim / d cons)^T gprim at the states with primitives prim. */
 int eb_cons_to_prim_vjp_f64(hipStream_t stream, const eb_prim_f64* prim, const eb_prim_f64* gprim,
                             const eb_cons_f64* gcons, int64_t n0, int64_t n1, int64_t n2, int index_bits,

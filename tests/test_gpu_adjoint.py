@@ -270,3 +270,36 @@ def test_implosion_gradient_matches_torch_reference_32(eb, ex):
     err = (ours - ref).abs().max().item() / ref.abs().max().item()
     assert err < 1e-10, err
     assert (ref[0].abs() > 0).sum() > 1000
+
+
+@pytest.mark.parametrize("first", [False, True])
+def test_vjp_region_is_exact_for_localised_adjoint(eb, ex, first):
+    """An adjoint step restricted to the support of mu_in grown by one cell equals
+    the full step bitwise (zero-filled outputs), for both step forms."""
+    from benchmark_euler_kernels_amd._lib import call
+
+    shape = (40, 30, 26)
+    P0 = _smooth_state(shape, 9)
+    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)])
+    g = torch.Generator().manual_seed(3)
+    mu = torch.zeros(5, shape[2], shape[1], shape[0], dtype=torch.float64)
+    mu[:, 8:13, 10:18, 5:21] = torch.randn(5, 5, 8, 16, generator=g, dtype=torch.float64)
+    mu = mu.cuda()
+    MU = eb.EulerConsArrays(*[mu[f] for f in range(5)])
+    dt = torch.tensor([2e-3], dtype=torch.float64, device="cuda")
+    ops = eb.operators
+    name = "eb_godunov_vjp_first_f64" if first else "eb_godunov_vjp_f64"
+    outs = []
+    # support: i 5..20, j 10..17, k 8..12 (array coordinates); grown by one cell
+    for region in (None, (ctypes.c_int64 * 6)(4, 21, 9, 18, 7, 13)):
+        out = torch.zeros_like(mu)
+        O = (eb.EulerPrimArrays if first else eb.EulerConsArrays)(*[out[f] for f in range(5)])
+        ptrs = ops.prim_ptrs(O) if first else ops.cons_ptrs(O)
+        call(name, torch.cuda.current_stream().cuda_stream, ctypes.byref(ops.prim_ptrs(prim)),
+             ctypes.byref(ops.cons_ptrs(MU)), ctypes.byref(ptrs), *shape, 32, ctypes.c_double(G),
+             ctypes.c_double(1 / 38), ctypes.c_double(1 / 28), ctypes.c_double(1 / 24), dt.data_ptr(),
+             ctypes.c_double(0.5), region)
+        torch.cuda.synchronize()
+        outs.append(out)
+    assert torch.equal(outs[0], outs[1])
+    assert outs[0].abs().sum() > 0
