@@ -99,6 +99,9 @@ def _signatures():
         sig[f"eb_step_fused_{suf}"] = (
             [_vp, _vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 5 + [_vp, _vp, _vp, _vp]
         )
+        sig[f"eb_step_fused_region_{suf}"] = (
+            [_vp, _vp, _P(PrimPtrs), _P(ConsPtrs), _P(PrimPtrs)] + ext + [real] * 5 + [_vp, _vp, _vp, _vp, _vp]
+        )
         sig[f"eb_init_implode_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 3
         sig[f"eb_init_diagonal_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 8
         sig[f"eb_init_random_{suf}"] = [_vp, _P(PrimPtrs), _i64, _int, _u64] + [_f64] * 5

@@ -12,6 +12,12 @@ bitwise the forward) and then reversed step by step through the hand-written HIP
 vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). The last segment's
 primitive states are kept by the forward pass itself. Memory: one state per
 checkpoint plus one primitive state per step of a segment.
+
+Locality: the adjoint of P_n is zero outside the support box of dL/dP_N grown by
+N - n cells (one cell per step, the stencil's reach). Each adjoint step only
+processes that box, and each recomputed step only advances the cells the adjoint
+step will read (the box grown by one more cell); both are exact, so a localised
+loss costs less than a full-grid one.
 """
 from __future__ import annotations
 
@@ -86,10 +92,14 @@ class Rollout:
              ctypes.byref(ops.prim_ptrs(dst)), *self._geom(), self._slots[1:].data_ptr())
 
     def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
-               dt_out: int):
-        call(f"eb_step_fused_{self.suf}", self.ex.context, self.ex.handle, ctypes.byref(ops.prim_ptrs(src)),
-             ctypes.byref(ops.cons_ptrs(cons)), ctypes.byref(ops.prim_ptrs(dst)), *self._geom(),
-             ops._real(self.suf, self.cfl), dt_in, self._slots[1:].data_ptr(), dt_out, None)
+               dt_out: int, region=None):
+        args = (self.ex.context, self.ex.handle, ctypes.byref(ops.prim_ptrs(src)), ctypes.byref(ops.cons_ptrs(cons)),
+                ctypes.byref(ops.prim_ptrs(dst)), *self._geom(), ops._real(self.suf, self.cfl), dt_in,
+                self._slots[1:].data_ptr(), dt_out, None)
+        if region is None:
+            call(f"eb_step_fused_{self.suf}", *args)
+        else:
+            call(f"eb_step_fused_region_{self.suf}", *args, region)
 
     # ---------------------------------------------------------------- forward
     def forward(self, prim0: ops.EulerPrimArrays, frozen_dts: Optional[torch.Tensor] = None) -> ops.EulerPrimArrays:
@@ -116,7 +126,7 @@ class Rollout:
         # The last segment's states are written straight into the segment buffers,
         # so the backward starts without recomputing it (and needs no checkpoint for it).
         self.tail = ((self.nsteps - 1) // self.k) * self.k
-        self._ensure_segment(cons, self.nsteps - self.tail)
+        self._ensure_segment(cons, self.nsteps - self.tail, valid=False)
         buf = [_clone_prim(prim0), _empty_prim_like(prim0)]
         self._ghost_refresh(cons, buf[1])
         self.checkpoints = {}
@@ -139,20 +149,30 @@ class Rollout:
         self._tail_ready = True
         return _clone_prim(self.prim_final)
 
-    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int):
+    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool):
+        """Grows the segment buffers to `need`. valid=True fills new buffers with a
+        valid state (P_0): a box-restricted recompute leaves the cells outside its
+        box as they were, and the adjoint steps read them with a zero adjoint
+        weight, which needs finite values. The forward pass writes every interior
+        cell of the buffers it asks for, so it passes valid=False."""
         while len(self._segment) < need:
-            b = _empty_prim_like(cons)
+            b = _clone_prim(self.prim0) if valid else _empty_prim_like(cons)
             self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
             self._segment.append(b)
 
     # --------------------------------------------------------------- backward
-    def _recompute(self, start: int, stop: int) -> List[ops.EulerPrimArrays]:
+    def _recompute(self, start: int, stop: int, needed=None) -> List[ops.EulerPrimArrays]:
         """P_n for n in [start, stop), recomputed from the checkpoint at `start`
         with the recorded time steps (bitwise the forward's states). The backward
         visits each segment once, so the checkpoint itself becomes the working state
-        (and its memory is released with the segment)."""
+        (and its memory is released with the segment).
+
+        needed(n) -> region (six int64, inclusive interior box) or None: P_n is
+        only required on that box. The boxes must shrink by at least one cell per
+        step (needed(n+1) grown by one cell lies inside needed(n)); P_n is then
+        exact on needed(n) and holds some earlier valid state elsewhere."""
         cons = self.checkpoints.pop(start)
-        self._ensure_segment(cons, stop - start)
+        self._ensure_segment(cons, stop - start, valid=True)
         if start == 0:
             prims = [self.prim0]
         else:
@@ -160,7 +180,8 @@ class Rollout:
             prims = [self._segment[0]]
         for n in range(start, stop - 1):
             nxt = self._segment[n - start + 1]
-            self._fused(prims[-1], cons, nxt, self.dts[n:].data_ptr(), self._slots[0:].data_ptr())
+            self._fused(prims[-1], cons, nxt, self.dts[n:].data_ptr(), self._slots[0:].data_ptr(),
+                        None if needed is None else needed(n + 1))
             prims.append(nxt)
         return prims
 
@@ -197,11 +218,18 @@ class Rollout:
         gprim0 = _zeros_prim_like(self.prim0)
         step = 0
 
-        def region():
+        def grown(g):
             if box is None:
                 return (ctypes.c_int64 * 6)(1, 0, 1, 0, 1, 0)  # empty: ghost cells only
-            g = step
             return (ctypes.c_int64 * 6)(box[0] - g, box[1] + g, box[2] - g, box[3] + g, box[4] - g, box[5] + g)
+
+        def region():
+            # the adjoint of P_n is nonzero at most on the box grown by N - n cells
+            return grown(step)
+
+        def needed(n):
+            # the adjoint step of P_n reads P_n on its output box grown by one cell
+            return grown(self.nsteps - n + 1)
         starts = sorted(self.checkpoints) + [self.tail]  # consumed segment by segment below
         for start in reversed(starts):
             stop = min(start + self.k, self.nsteps)
@@ -210,7 +238,7 @@ class Rollout:
                 head = self.prim0 if start == 0 else self._segment[0]
                 prims = [head] + self._segment[1:stop - start]
             else:
-                prims = self._recompute(start, stop)
+                prims = self._recompute(start, stop, needed)
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
                 dt = self.dts[n:n + 1].data_ptr()

@@ -246,6 +246,41 @@ void launch_tiled(
     }
 }
 
+// The fused step of the 64 x 8 tiled kernel on the box of interior cells [lo, hi]
+// (inclusive, per axis): the tiles that meet the box are marched over the box's
+// planes and only the box's cells are written. Used where only part of the grid
+// has to advance (the adjoint's recompute of a localised adjoint).
+template <class T, class I>
+int launch_fused_region(hipStream_t s, eb::PrimPtrs<T const> const& pc, eb::ConsPtrs<T> const& cu,
+                        eb::PrimPtrs<T> const& pout, int64_t n0, int64_t n1, int64_t n2, int64_t const* lo,
+                        int64_t const* hi, T gamma, T ds0, T ds1, T ds2, eb::DtArg<T> const& dta, T invdx0, T invdx1,
+                        T invdx2, eb::DtSink<T> const& sink)
+{
+    constexpr int BY = 8;
+    int64_t const x0 = (lo[0] - 1) / 64;
+    int64_t const x1 = (hi[0] - 1) / 64 + 1;
+    int64_t const y0 = (lo[1] - 1) / BY;
+    int64_t const y1 = (hi[1] - 1) / BY + 1;
+    int64_t const columns = (x1 - x0) * (y1 - y0);
+    int64_t kc = (hi[2] - lo[2] + 1) * columns / 512;
+    kc = kc < 8 ? 8 : (kc > 128 ? 128 : kc);
+    int const kchunk = static_cast<int>(kc < n2 - 2 ? kc : n2 - 2);
+    int64_t const z0 = (lo[2] - 1) / kchunk;
+    int64_t const z1 = (hi[2] - 1) / kchunk + 1;
+    int64_t const nblk = columns * (z1 - z0);
+    if (nblk > std::numeric_limits<int32_t>::max() / 2) {
+        return fail(EB_ERR_INDEX_OVERFLOW, "tile count exceeds the grid limit");
+    }
+    eb::CellBox<I> const box {static_cast<I>(lo[0]), static_cast<I>(hi[0]), static_cast<I>(lo[1]),
+                              static_cast<I>(hi[1]), static_cast<I>(lo[2]), static_cast<I>(hi[2])};
+    hipLaunchKernelGGL((eb::k_godunov_box<T, I, BY>), dim3(static_cast<unsigned int>(nblk)), dim3(64, BY), 0, s, pc,
+                       cu, pout, static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk,
+                       static_cast<int>(x1 - x0), static_cast<int>(y1 - y0), static_cast<int>(z1 - z0),
+                       static_cast<int>(x0), static_cast<int>(y0), static_cast<int>(z0), box, gamma,
+                       static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2, sink);
+    return check_launch("step_fused_region");
+}
+
 // Planes marched per block when the configured kchunk is 0. A block re-reads one
 // primitive plane below its chunk, so long chunks save traffic; the chunk is
 // shortened until the launch has about 512 blocks (two per CU) so that small
@@ -414,7 +449,8 @@ int run_step_fused(
         T const* dt_in,
         T const* ghost_min,
         T* dt_out,
-        double* time_acc)
+        double* time_acc,
+        int64_t const* region)
 {
     T ds0;
     T ds1;
@@ -425,6 +461,22 @@ int run_step_fused(
     auto const p = prim_ptrs<T>(prim_in);
     eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
     eb::DtSink<T> const sink {ctx->scratch, dt_out, ghost_min, time_acc, dt_in, cfl};
+    if (region != nullptr) {
+        int64_t lo[3] = {1, 1, 1};
+        int64_t hi[3] = {n0 - 2, n1 - 2, n2 - 2};
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = region[2 * d] > lo[d] ? region[2 * d] : lo[d];
+            hi[d] = region[2 * d + 1] < hi[d] ? region[2 * d + 1] : hi[d];
+        }
+        if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) {
+            return EB_OK; // nothing to advance
+        }
+        if (static_cast<__int128>(n0) * n1 * static_cast<int64_t>(sizeof(T)) >= (static_cast<__int128>(1) << 32)) {
+            return fail(EB_ERR_UNSUPPORTED, "a k-plane must hold fewer than 2^32 bytes per field");
+        }
+        return launch_fused_region<T, I>(s, pc, cons_ptrs<T>(cons), prim_ptrs<T>(prim_out), n0, n1, n2, lo, hi,
+                                         gamma, ds0, ds1, ds2, dta, T(1) / dx0, T(1) / dx1, T(1) / dx2, sink);
+    }
     return dispatch_tiled<T, I, true>(s, pc, cons_ptrs<T>(cons), prim_ptrs<T>(prim_out), n0, n1, n2, gamma, ds0,
                                       ds1, ds2, dta, T(1) / dx0, T(1) / dx1, T(1) / dx2, sink);
 }
@@ -664,7 +716,7 @@ int abi_ghost_refresh(eb_context* ctx, hipStream_t s, CS const* cons, PS const* 
 template <class T, class PS, class CS>
 int abi_step_fused(eb_context* ctx, hipStream_t s, PS const* prim_in, CS const* cons, PS const* prim_out, int64_t n0,
                    int64_t n1, int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T cfl, T const* dt_in,
-                   T const* ghost_min, T* dt_out, double* time_acc)
+                   T const* ghost_min, T* dt_out, double* time_acc, int64_t const* region = nullptr)
 {
     EB_CHECK(check_extents(n0, n1, n2, bits));
     if (n0 < 3 || n1 < 3 || n2 < 3) {
@@ -678,9 +730,9 @@ int abi_step_fused(eb_context* ctx, hipStream_t s, PS const* prim_in, CS const* 
         return fail(EB_ERR_INVALID_ARGUMENT, "prim_out / dt_out must not alias the inputs");
     }
     return bits == 32 ? run_step_fused<T, int32_t>(ctx, s, prim_in, cons, prim_out, n0, n1, n2, gamma, dx0, dx1,
-                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc)
+                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc, region)
                       : run_step_fused<T, int64_t>(ctx, s, prim_in, cons, prim_out, n0, n1, n2, gamma, dx0, dx1,
-                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc);
+                                                   dx2, cfl, dt_in, ghost_min, dt_out, time_acc, region);
 }
 
 template <class T, class I, class PS>
@@ -1072,6 +1124,14 @@ extern "C" {
     {                                                                                                                \
         return abi_step_fused<T>(ctx, s, pin, c, pout, n0, n1, n2, bits, gamma, dx0, dx1, dx2, cfl, dt_in,          \
                                  ghost_min, dt_out, time_acc);                                                       \
+    }                                                                                                                \
+    int eb_step_fused_region_##SUF(eb_context* ctx, hipStream_t s, const eb_prim_##SUF* pin, const eb_cons_##SUF* c,\
+                                   const eb_prim_##SUF* pout, int64_t n0, int64_t n1, int64_t n2, int bits, T gamma,\
+                                   T dx0, T dx1, T dx2, T cfl, const T* dt_in, const T* ghost_min, T* dt_out,      \
+                                   double* time_acc, const int64_t* region)                                          \
+    {                                                                                                                \
+        return abi_step_fused<T>(ctx, s, pin, c, pout, n0, n1, n2, bits, gamma, dx0, dx1, dx2, cfl, dt_in,          \
+                                 ghost_min, dt_out, time_acc, region);                                               \
     }                                                                                                                \
     int eb_init_implode_##SUF(hipStream_t s, const eb_prim_##SUF* p, int64_t n0, int64_t n1, int64_t n2, int bits,  \
                               T dx0, T dx1, T dx2)                                                                   \

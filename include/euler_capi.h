@@ -144,6 +144,22 @@ int eb_step_fused_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* pr
                       const eb_prim_f32* prim_out, int64_t n0, int64_t n1, int64_t n2, int index_bits,
                       float gamma, float dx0, float dx1, float dx2, float cfl,
                       const float* dt_in, const float* ghost_min, float* dt_out, double* time_acc);
+/* The same step on the box of interior cells region = {i_lo, i_hi, j_lo, j_hi,
+ * k_lo, k_hi} (inclusive, clipped to the interior): only the box's cells are
+ * written, the others keep their values, and dt_out is the minimum over the box.
+ * Exact on the box when prim_in is exact on the box grown by one cell and cons on
+ * the box. For callers that only need part of the grid advanced, such as the
+ * adjoint's recompute of a localised adjoint. No reference counterpart. */
+int eb_step_fused_region_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64* prim_in,
+                             const eb_cons_f64* cons, const eb_prim_f64* prim_out, int64_t n0, int64_t n1,
+                             int64_t n2, int index_bits, double gamma, double dx0, double dx1, double dx2,
+                             double cfl, const double* dt_in, const double* ghost_min, double* dt_out,
+                             double* time_acc, const int64_t* region);
+int eb_step_fused_region_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim_in,
+                             const eb_cons_f32* cons, const eb_prim_f32* prim_out, int64_t n0, int64_t n1,
+                             int64_t n2, int index_bits, float gamma, float dx0, float dx1, float dx2, float cfl,
+                             const float* dt_in, const float* ghost_min, float* dt_out, double* time_acc,
+                             const int64_t* region);
 
 /* ---- init_implode.hpp:7-45 (sphere r < 0.25 about the centre) ---------- */
 int eb_init_implode_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n0, int64_t n1, int64_t n2,

@@ -303,3 +303,65 @@ def test_vjp_region_is_exact_for_localised_adjoint(eb, ex, first):
         outs.append(out)
     assert torch.equal(outs[0], outs[1])
     assert outs[0].abs().sum() > 0
+
+
+def test_step_fused_region_is_exact_on_its_box(eb, ex):
+    """The box-restricted fused step (eb_step_fused_region_f64) writes exactly the
+    box's cells, bitwise equal to the full step's, and leaves every other cell of
+    the output primitives and of the conservatives as it was; its dt is the
+    minimum over the box (not below the full step's)."""
+    from benchmark_euler_kernels_amd._lib import call
+
+    ops = eb.operators
+    shape = (150, 21, 23)  # three 64-wide tiles in i, ragged in every axis
+    dx = (1 / 148, 1 / 19, 1 / 21)
+    P0 = _smooth_state(shape, 4)
+    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)])
+    cons0 = eb.EulerConsArrays(*[torch.empty_like(t) for t in prim.tensors()])
+    ops.prim_to_cons(ex, prim, cons0, eb.PerfectGas(G))
+    dt_in = torch.tensor([1e-3], dtype=torch.float64, device="cuda")
+    big = torch.tensor([1e300], dtype=torch.float64, device="cuda")
+    box = (60, 131, 3, 12, 5, 19)  # i_lo, i_hi, j_lo, j_hi, k_lo, k_hi
+    sentinel = [torch.full_like(t, 7.0) for t in prim.tensors()]
+    results = []
+    for region in (None, (ctypes.c_int64 * 6)(*box)):
+        cons = eb.EulerConsArrays(*[t.clone() for t in cons0.tensors()])
+        out = eb.EulerPrimArrays(*[t.clone() for t in sentinel])
+        dt_out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        args = (ex.context, ex.handle, ctypes.byref(ops.prim_ptrs(prim)), ctypes.byref(ops.cons_ptrs(cons)),
+                ctypes.byref(ops.prim_ptrs(out)), *shape, 32, ctypes.c_double(G), *map(ctypes.c_double, dx),
+                ctypes.c_double(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr(), None)
+        if region is None:
+            call("eb_step_fused_f64", *args)
+        else:
+            call("eb_step_fused_region_f64", *args, region)
+        torch.cuda.synchronize()
+        results.append((torch.stack(cons.tensors()), torch.stack(out.tensors()), dt_out.item()))
+    (c_full, p_full, dt_full), (c_box, p_box, dt_box) = results
+    inside = torch.zeros(shape[2], shape[1], shape[0], dtype=torch.bool, device="cuda")
+    inside[box[4]:box[5] + 1, box[2]:box[3] + 1, box[0]:box[1] + 1] = True
+    assert torch.equal(p_box[:, inside], p_full[:, inside])
+    assert torch.equal(c_box[:, inside], c_full[:, inside])
+    assert torch.equal(p_box[:, ~inside], torch.stack(sentinel)[:, ~inside])
+    assert torch.equal(c_box[:, ~inside], torch.stack(cons0.tensors())[:, ~inside])
+    assert dt_box >= dt_full > 0
+
+
+@pytest.mark.parametrize("nsteps,k", [(10, 4), (12, 5)])
+def test_localised_rollout_gradient_is_bitwise_checkpoint_invariant(eb, ex, nsteps, k):
+    """A loss over a small corner box of a 70 x 40 x 36 grid: the backward with
+    box-restricted recomputes (checkpoints every k steps; the last segment shorter
+    than k, so recompute buffers are added in the backward) equals the backward
+    of the fully stored rollout bitwise."""
+    shape = (72, 42, 38)
+    dx = (1 / 70, 1 / 40, 1 / 36)
+    P0 = _smooth_state(shape, 6)
+    mask = torch.zeros(shape[2], shape[1], shape[0], dtype=torch.bool, device="cuda")
+    mask[30:35, 2:6, 60:68] = True
+    grads = []
+    for kk in (nsteps, k):
+        ts, out = _hip_grad(eb, ex, P0, dx, nsteps, kk)
+        (out[0] * out[0] * mask).sum().backward()
+        grads.append(torch.stack([t.grad for t in ts]))
+    assert torch.equal(grads[0], grads[1])
+    assert grads[0].abs().sum() > 0
