@@ -348,9 +348,11 @@ def profiled_traffic(kernel: str, n: int):
     return rec["traffic_bytes_per_launch"] if rec.get("kernel") == kernel else None
 
 
-def kernel_name(cfg) -> str:
-    variant, by, _ = cfg
-    kind = {0: "k_godunov_faces", 1: "k_godunov_tiled", 2: "k_godunov_stream"}[variant]
+def kernel_name(eb) -> str:
+    """The FP64 flux kernel the current configuration launches (int64 indices)."""
+    variant = eb.get_godunov_config()[0]
+    by = eb.godunov_tile_height(fused=False)
+    kind = {0: "k_godunov_faces", 1: "k_godunov_tiled", 2: "k_godunov_stream", 3: "k_godunov_linear"}[variant]
     return f"eb::{kind}<double, long, {by}, false>"
 
 
@@ -524,8 +526,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(kernel_name(eb.get_godunov_config()), n),
-            "kernel": kernel_name(eb.get_godunov_config()),
+            "traffic": profiled_traffic(kernel_name(eb), n),
+            "kernel": kernel_name(eb),
             "bytes_per_launch": bytes_godunov,
             "avg_launch_ms": round(ms, 4),
         },

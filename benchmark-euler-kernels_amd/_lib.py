@@ -73,6 +73,7 @@ def _signatures():
         "eb_context_destroy": [_vp],
         "eb_set_godunov_config": [_int, _int, _int],
         "eb_get_godunov_config": [_P(_int), _P(_int), _P(_int)],
+        "eb_get_godunov_tile_height": [_int, _int, _P(_int)],
         "eb_batch_create_f64": [_P(_vp), _P(BatchInstanceF64), _int, _int],
         "eb_batch_create_f32": [_P(_vp), _P(BatchInstanceF32), _int, _int],
         "eb_batch_destroy": [_vp],

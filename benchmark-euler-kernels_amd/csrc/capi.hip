@@ -29,13 +29,30 @@ thread_local std::string g_last_error;
 // per block. Variant 0 = face-parallel waves, 1 = cell-parallel waves (default,
 // fastest measured at 512^3 on MI355X), 2 = streamed with a producer wave.
 std::atomic<int> g_variant {1};
-std::atomic<int> g_tile_by {8};
+std::atomic<int> g_tile_by {0};     // 0: chosen per kernel by auto_by
 std::atomic<int> g_tile_kchunk {0}; // 0: sized per launch by auto_kchunk
 
 bool config_supported(int variant, int by)
 {
-    return (variant == 0 && (by == 2 || by == 4)) || (variant == 1 && (by == 4 || by == 8 || by == 16))
-           || (variant == 2 && (by == 4 || by == 8)) || (variant == 3 && (by == 4 || by == 8));
+    return (variant >= 0 && variant <= 3 && by == 0) || (variant == 0 && (by == 2 || by == 4))
+           || (variant == 1 && (by == 4 || by == 8 || by == 16)) || (variant == 2 && (by == 4 || by == 8))
+           || (variant == 3 && (by == 4 || by == 8));
+}
+
+// Tile height when the configuration leaves it open (by = 0), from MI355X timings
+// at 512^3. The FP64 flux kernel of the 64 x by tiles needs ~158 VGPRs: three
+// waves per SIMD, which 4-wave blocks (three per CU) can use and 8-wave blocks
+// (one per CU) cannot, so it takes by = 4. Its fused form (~172 VGPRs, two waves
+// per SIMD) and the FP32 kernels (~94-108 VGPRs) run best with by = 8.
+int auto_by(int variant, std::size_t real_bytes, bool fused)
+{
+    if (variant == 0) {
+        return 4;
+    }
+    if (variant == 1 && real_bytes == 8 && !fused) {
+        return 4;
+    }
+    return 8;
 }
 
 int fail(int status, std::string const& msg)
@@ -314,7 +331,8 @@ int dispatch_tiled(
         eb::DtSink<T> const& sink)
 {
     int const variant = g_variant.load();
-    int const by = g_tile_by.load();
+    int const by_set = g_tile_by.load();
+    int const by = by_set > 0 ? by_set : auto_by(variant, sizeof(T), Fused);
     int const kc_set = g_tile_kchunk.load();
     int const kchunk = kc_set > 0 ? kc_set : auto_kchunk(n0, n1, n2, by);
     // The block count must fit a 32-bit grid and a plane must fit 32-bit offsets.
@@ -598,11 +616,23 @@ int eb_set_godunov_config(int variant, int by, int kchunk)
         return fail(EB_ERR_INVALID_ARGUMENT, "kchunk must be >= 0 (0: automatic)");
     }
     if (!config_supported(variant, by)) {
-        return fail(EB_ERR_UNSUPPORTED, "unsupported (variant, tile height): (0, 2|4), (1, 4|8|16), (2, 4|8), (3, 4|8)");
+        return fail(EB_ERR_UNSUPPORTED,
+                    "unsupported (variant, tile height): (0-3, 0 = automatic), (0, 2|4), (1, 4|8|16), (2, 4|8), "
+                    "(3, 4|8)");
     }
     g_variant.store(variant);
     g_tile_by.store(by);
     g_tile_kchunk.store(kchunk);
+    return EB_OK;
+}
+
+int eb_get_godunov_tile_height(int real_bytes, int fused, int* by)
+{
+    if (by == nullptr || (real_bytes != 4 && real_bytes != 8)) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "real_bytes must be 4 or 8 and by non-null");
+    }
+    int const set = g_tile_by.load();
+    *by = set > 0 ? set : auto_by(g_variant.load(), static_cast<std::size_t>(real_bytes), fused != 0);
     return EB_OK;
 }
 

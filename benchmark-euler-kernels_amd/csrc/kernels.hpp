@@ -535,19 +535,17 @@ __device__ __forceinline__ T godunov_tile(
         }
         lds_barrier();
 
-        // B. faces.
-        T const c_next = sound_speed(gamma, q_next.d, q_next.p);
-        T const e_next = total_energy(q_next, gm1);
-        Flux<T> const fx_lo
-                = hllc_flux<0>(cell(ty + 1, tx), sq[5][ty + 1][tx], sq[6][ty + 1][tx], q_cur, c_cur, e_cur);
-        Flux<T> const fy_lo
-                = hllc_flux<1>(cell(ty, tx + 1), sq[5][ty][tx + 1], sq[6][ty][tx + 1], q_cur, c_cur, e_cur);
-        Flux<T> const fz_hi = hllc_flux<2>(q_cur, c_cur, e_cur, q_next, c_next, e_next);
-        sfy[0][ty][tx] = fy_lo.d;
-        sfy[1][ty][tx] = fy_lo.e;
-        sfy[2][ty][tx] = fy_lo.m0;
-        sfy[3][ty][tx] = fy_lo.m1;
-        sfy[4][ty][tx] = fy_lo.m2;
+        // B. faces, in an order that keeps few face results live while another
+        // face is solved: y (kept in LDS only), the tile's high-side y and x faces,
+        // z, and x last (exchanged by shuffle right away).
+        {
+            Flux<T> const f = hllc_flux<1>(cell(ty, tx + 1), sq[5][ty][tx + 1], sq[6][ty][tx + 1], q_cur, c_cur, e_cur);
+            sfy[0][ty][tx] = f.d;
+            sfy[1][ty][tx] = f.e;
+            sfy[2][ty][tx] = f.m0;
+            sfy[3][ty][tx] = f.m1;
+            sfy[4][ty][tx] = f.m2;
+        }
         if (ty == BY - 1) {
             // High-side y faces of the tile: rows BY and BY+1 of the staged plane.
             Flux<T> const f = hllc_flux<1>(cell(BY, tx + 1), sq[5][BY][tx + 1], sq[6][BY][tx + 1],
@@ -569,18 +567,24 @@ __device__ __forceinline__ T godunov_tile(
             sfx[3][tx] = f.m1;
             sfx[4][tx] = f.m2;
         }
-        lds_barrier();
-
-        // C. combine and update.
+        T const c_next = sound_speed(gamma, q_next.d, q_next.p);
+        T const e_next = total_energy(q_next, gm1);
+        Flux<T> const fz_hi = hllc_flux<2>(q_cur, c_cur, e_cur, q_next, c_next, e_next);
+        Flux<T> const fx_lo
+                = hllc_flux<0>(cell(ty + 1, tx), sq[5][ty + 1][tx], sq[6][ty + 1][tx], q_cur, c_cur, e_cur);
         Flux<T> fx_hi;
         fx_hi.d = __shfl_down(fx_lo.d, 1, 64);
         fx_hi.e = __shfl_down(fx_lo.e, 1, 64);
         fx_hi.m0 = __shfl_down(fx_lo.m0, 1, 64);
         fx_hi.m1 = __shfl_down(fx_lo.m1, 1, 64);
         fx_hi.m2 = __shfl_down(fx_lo.m2, 1, 64);
+        lds_barrier();
+
+        // C. combine (x, y, z in the reference's order) and update.
         if (tx == BX - 1) {
             fx_hi = Flux<T> {sfx[0][ty], sfx[1][ty], sfx[2][ty], sfx[3][ty], sfx[4][ty]};
         }
+        Flux<T> const fy_lo {sfy[0][ty][tx], sfy[1][ty][tx], sfy[2][ty][tx], sfy[3][ty][tx], sfy[4][ty][tx]};
         Flux<T> const fy_hi {sfy[0][ty + 1][tx], sfy[1][ty + 1][tx], sfy[2][ty + 1][tx], sfy[3][ty + 1][tx],
                              sfy[4][ty + 1][tx]};
         Flux<T> acc {T(0), T(0), T(0), T(0), T(0)};

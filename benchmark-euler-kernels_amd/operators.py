@@ -369,9 +369,10 @@ def boundary_flux(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, eos: Perf
 
 
 def set_godunov_config(variant: int, by: int, kchunk: int) -> None:
-    """Godunov kernel variant (0 face-parallel, 1 cell-parallel, 2 streamed), tile
-    height and planes marched per block (0: sized per launch from the grid).
-    Results do not depend on it."""
+    """Godunov kernel variant (0 face-parallel, 1 cell-parallel, 2 streamed,
+    3 linear tiles), tile height (0: chosen per kernel) and planes marched per
+    block (0: sized per launch from the grid). Default (1, 0, 0). Results do not
+    depend on it."""
     call("eb_set_godunov_config", variant, by, kchunk)
 
 
@@ -381,10 +382,18 @@ def get_godunov_config():
     return v.value, by.value, kc.value
 
 
+def godunov_tile_height(dtype=torch.float64, fused: bool = False) -> int:
+    """Tile height the current configuration launches for this real type, in the
+    flux kernel or (fused=True) the fused step."""
+    by = ctypes.c_int()
+    call("eb_get_godunov_tile_height", 8 if dtype == torch.float64 else 4, int(fused), ctypes.byref(by))
+    return by.value
+
+
 __all__ = [
     "PerfectGas", "UniformMesh3d", "hllc", "EulerPrimArrays", "EulerConsArrays", "ExecSpace",
     "create_prim_arrays_1d", "create_cons_arrays_1d", "to_mdspan", "as_const", "size", "size_bytes",
     "cons_to_prim", "prim_to_cons", "time_step", "time_step_async", "godunov", "init_implode",
-    "init_diagonal", "init_random", "set_godunov_config", "get_godunov_config", "extents",
+    "init_diagonal", "init_random", "set_godunov_config", "get_godunov_config", "godunov_tile_height", "extents",
     "conserved_totals", "boundary_flux",
 ]

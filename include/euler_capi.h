@@ -62,12 +62,17 @@ const char* eb_last_error(void);
  *   variant 3: linear tiles of 64*by consecutive cells of a plane (rows wrap, no
  *              lane lost to ragged x extents), `by` in {4, 8}; needs n0 <= 64*by
  *              (else EB_ERR_UNSUPPORTED);
+ *   `by` 0 (any variant): the tile height is chosen per kernel (variant 1: 4 for
+ *   the FP64 flux kernel, 8 for its fused form and for FP32);
  *   `kchunk` >= 1 planes marched per block along k, or 0: chosen per launch from
  *   the grid (long chunks, shortened until the launch has ~512 blocks).
- *   Defaults (1, 8, 0).
+ *   Defaults (1, 0, 0).
  * All configurations give bitwise identical results. */
 int eb_set_godunov_config(int variant, int by, int kchunk);
 int eb_get_godunov_config(int* variant, int* by, int* kchunk);
+/* The tile height the current configuration launches for reals of real_bytes
+ * (4 or 8) in the flux kernel (fused = 0) or the fused step (fused = 1). */
+int eb_get_godunov_tile_height(int real_bytes, int fused, int* by);
 
 /* ---- cons_to_prim.hpp:7-33 --------------------------------------------- */
 int eb_cons_to_prim_f64(hipStream_t stream, const eb_cons_f64* cons, const eb_prim_f64* prim,
