@@ -153,8 +153,10 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
         return (time.perf_counter() - t0) * 1e3, r
 
     def fwd_only():
-        with torch.no_grad():
-            return eb.adjoint.rollout(clone_prim(), eos, mesh, 0.5, nsteps, k, exec_space=ex)
+        # the differentiable forward (checkpoints and the last segment kept), no backward
+        ts = [t.clone().requires_grad_(True) for t in P0.tensors()]
+        return eb.adjoint.rollout(eb.EulerPrimArrays(*ts, index_bits=P0.index_bits), eos, mesh, 0.5, nsteps, k,
+                                  exec_space=ex)
 
     def fwd_bwd():
         ts = [t.clone().requires_grad_(True) for t in P0.tensors()]

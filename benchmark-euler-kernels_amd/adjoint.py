@@ -102,10 +102,12 @@ class Rollout:
             call(f"eb_step_fused_region_{self.suf}", *args, region)
 
     # ---------------------------------------------------------------- forward
-    def forward(self, prim0: ops.EulerPrimArrays, frozen_dts: Optional[torch.Tensor] = None) -> ops.EulerPrimArrays:
+    def forward(self, prim0: ops.EulerPrimArrays, frozen_dts: Optional[torch.Tensor] = None,
+                keep: bool = True) -> ops.EulerPrimArrays:
         """Runs the rollout from P_0; returns P_N (new tensors). With frozen_dts
         (raw time steps of an earlier rollout, one per step) those time steps are
-        used instead of the CFL ones, which is the map the adjoint differentiates."""
+        used instead of the CFL ones, which is the map the adjoint differentiates.
+        keep=False stores neither checkpoints nor states (no backward follows)."""
         self.suf = ops._validate(prim0)
         if min(ops.extents(prim0)) < 3:
             raise ValueError("a rollout needs at least one interior cell")
@@ -125,14 +127,14 @@ class Rollout:
         self.dts = dt[:self.nsteps]
         # The last segment's states are written straight into the segment buffers,
         # so the backward starts without recomputing it (and needs no checkpoint for it).
-        self.tail = ((self.nsteps - 1) // self.k) * self.k
+        self.tail = ((self.nsteps - 1) // self.k) * self.k if keep else self.nsteps
         self._ensure_segment(cons, self.nsteps - self.tail, valid=False)
         buf = [_clone_prim(prim0), _empty_prim_like(prim0)]
         self._ghost_refresh(cons, buf[1])
         self.checkpoints = {}
         cur = buf[0]
         for n in range(self.nsteps):
-            if n % self.k == 0 and n != self.tail:
+            if keep and n % self.k == 0 and n != self.tail:
                 self.checkpoints[n] = _clone_cons(cons)
             m = n + 1 - self.tail
             if 0 <= m < self.nsteps - self.tail:
@@ -146,7 +148,7 @@ class Rollout:
                 self._ghost_refresh(cons, buf[0])
             cur = nxt
         self.prim_final = cur
-        self._tail_ready = True
+        self._tail_ready = keep
         return _clone_prim(self.prim_final)
 
     def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool):
@@ -260,7 +262,8 @@ class _RolloutFunction(torch.autograd.Function):
     def forward(ctx, rollout: Rollout, index_bits: int, frozen_dts, d, p, u0, u1, u2):
         P0 = ops.EulerPrimArrays(d.contiguous(), p.contiguous(), u0.contiguous(), u1.contiguous(), u2.contiguous(),
                                  index_bits=index_bits)
-        PN = rollout.forward(P0, frozen_dts)
+        # without a gradient to propagate, nothing is kept for a backward pass
+        PN = rollout.forward(P0, frozen_dts, keep=any(ctx.needs_input_grad[3:]))
         ctx.rollout = rollout
         return PN.tensors()
 

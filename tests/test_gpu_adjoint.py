@@ -365,3 +365,24 @@ def test_localised_rollout_gradient_is_bitwise_checkpoint_invariant(eb, ex, nste
         grads.append(torch.stack([t.grad for t in ts]))
     assert torch.equal(grads[0], grads[1])
     assert grads[0].abs().sum() > 0
+
+
+def test_rollout_without_grad_keeps_nothing_and_matches(eb, ex):
+    """Under no_grad the rollout stores no checkpoints or states, and its result
+    is bitwise that of the differentiable rollout."""
+    shape = (18, 14, 12)
+    dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
+    P0 = _smooth_state(shape, 3)
+    tape = []
+    ts, out = _hip_grad(eb, ex, P0, dx, 9, 4, tape=tape)
+    assert len(tape[0].checkpoints) == 2 and len(tape[0]._segment) == 1
+    tape2 = []
+    with torch.no_grad():
+        prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)], index_bits=32)
+        out2 = eb.adjoint.rollout(prim, eb.PerfectGas(G), eb.UniformMesh3d(*dx), 0.5, 9, 4, exec_space=ex,
+                                  tape=tape2)
+    assert not tape2[0].checkpoints and not tape2[0]._segment
+    for a, b in zip(out, out2):
+        assert torch.equal(a.detach(), b)
+    with pytest.raises(RuntimeError):
+        tape2[0].backward([torch.zeros_like(t) for t in out2])
