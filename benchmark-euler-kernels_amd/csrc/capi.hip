@@ -765,6 +765,22 @@ int abi_step_fused(eb_context* ctx, hipStream_t s, PS const* prim_in, CS const* 
                                                    dx2, cfl, dt_in, ghost_min, dt_out, time_acc, region);
 }
 
+template <class T>
+int abi_fill(hipStream_t s, T* a, int64_t n, T v)
+{
+    if (n < 0) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "negative element count");
+    }
+    if (n == 0) {
+        return EB_OK;
+    }
+    if (a == nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    hipLaunchKernelGGL((eb::k_fill<T>), dim3(pointwise_grid(n)), dim3(256), 0, s, a, n, v);
+    return check_launch("fill");
+}
+
 template <class T, class I, class PS>
 void launch_implode(hipStream_t s, PS const* prim, int64_t n0, int64_t n1, int64_t n2, T dx0, T dx1, T dx2)
 {
@@ -1172,6 +1188,10 @@ extern "C" {
                                T dx0, T dx1, T dx2, T offset, T d_hi, T p_hi, T d_lo, T p_lo)                        \
     {                                                                                                                \
         return abi_init_diagonal<T>(s, p, n0, n1, n2, bits, dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo);         \
+    }                                                                                                                \
+    int eb_fill_##SUF(hipStream_t s, T* a, int64_t n, T v)                                                          \
+    {                                                                                                                \
+        return abi_fill<T>(s, a, n, v);                                                                              \
     }                                                                                                                \
     int eb_init_random_##SUF(hipStream_t s, const eb_prim_##SUF* p, int64_t n, int bits, uint64_t seed,             \
                              double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma)                     \

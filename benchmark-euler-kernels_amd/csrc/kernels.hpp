@@ -103,6 +103,17 @@ __global__ __launch_bounds__(256) void k_time_step(
 
 // ------------------------------------------------------------------- init ---
 
+// Every element of one field set to a value (the init_from_value role of
+// benchmarks/benchmark_utils.hpp:19-44), one grid-stride store per element.
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T* a, int64_t n, T v)
+{
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * 256;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; idx < n; idx += stride) {
+        a[idx] = v;
+    }
+}
+
 // init_implode.hpp:16-43: sphere r^2 < 0.25^2 about the cube centre; the centre
 // coordinate is formed in double and rounded to T; ghosts included.
 template <class T, class I>

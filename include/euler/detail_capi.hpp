@@ -34,6 +34,7 @@ struct capi;
         static constexpr auto init_implode = &eb_init_implode_##SUF;            \
         static constexpr auto init_diagonal = &eb_init_diagonal_##SUF;          \
         static constexpr auto init_random = &eb_init_random_##SUF;              \
+        static constexpr auto fill = &eb_fill_##SUF;                            \
     };
 EB_CAPI_TRAITS(double, f64)
 EB_CAPI_TRAITS(float, f32)

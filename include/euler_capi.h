@@ -188,6 +188,10 @@ int eb_init_random_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n, i
                        double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma);
 int eb_init_random_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n, int index_bits, uint64_t seed,
                        double d_lo, double d_hi, double p_lo, double p_hi, double u_sigma);
+/* ---- benchmarks/benchmark_utils.hpp:19-44 (init_from_value) ---------------
+ * a[0..n) = value on the device, for one field of an array set. */
+int eb_fill_f64(hipStream_t stream, double* a, int64_t n, double value);
+int eb_fill_f32(hipStream_t stream, float* a, int64_t n, float value);
 
 
 /* ---- adjoint (no reference counterpart; the reference has no derivatives) ---

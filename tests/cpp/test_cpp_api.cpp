@@ -3,11 +3,13 @@
 // time_step = 1/(3c/dx)), the reference-order loop against the fused step loop
 // (eager and graph-replayed, bitwise), real/index type instantiations and the
 // error behaviour. Prints one line per check; exit status 0 iff all pass.
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <iostream>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -187,6 +189,50 @@ void npy_check()
     expect(ok && last == 23.5, "save_npy header and payload");
 }
 
+// benchmark_utils.hpp counterparts: int_cast range checks, init_from_value and
+// init_from_state fill every element (ghosts included) with the value.
+void harness_checks()
+{
+    expect(eb::int_cast<int>(std::int64_t(2147483647)) == 2147483647, "int_cast keeps an in-range value");
+    bool threw = false;
+    try {
+        (void)eb::int_cast<int>(std::int64_t(2147483648LL));
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    expect(threw, "int_cast throws past the target range");
+    threw = false;
+    try {
+        (void)eb::int_cast<unsigned>(-1);
+    } catch (std::runtime_error const&) {
+        threw = true;
+    }
+    expect(threw, "int_cast throws on a negative value for an unsigned target");
+
+    eb::HipExecSpace const ex;
+    auto const pa = eb::create_prim_arrays_1d<float>(ex, 5 * 6 * 7);
+    auto const prim = eb::to_mdspan<view_t<float, int>>(pa, 5, 6, 7);
+    auto const ca = eb::create_cons_arrays_1d<float>(ex, 5 * 6 * 7);
+    auto const cons = eb::to_mdspan<view_t<float, int>>(ca, 5, 6, 7);
+    eb::init_from_state(ex, prim, eb::EulerPrim<float> {1.5f, 2.5f, -0.25f, 0.5f, 3.0f});
+    eb::init_from_state(ex, cons, eb::EulerCons<float> {1.f, 2.5f, 0.f, -1.f, 7.f});
+    ex.fence();
+    bool ok = true;
+    float const pv[5] = {1.5f, 2.5f, -0.25f, 0.5f, 3.0f};
+    float const cv[5] = {1.f, 2.5f, 0.f, -1.f, 7.f};
+    auto const pf = std::array {prim.d, prim.p, prim.ux0, prim.ux1, prim.ux2};
+    auto const cf = std::array {cons.d, cons.e, cons.mx0, cons.mx1, cons.mx2};
+    for (int f = 0; f < 5; ++f) {
+        for (float x : eb::to_host(ex, pf[f])) {
+            ok = ok && x == pv[f];
+        }
+        for (float x : eb::to_host(ex, cf[f])) {
+            ok = ok && x == cv[f];
+        }
+    }
+    expect(ok, "init_from_state fills all 210 cells of every field");
+}
+
 } // namespace
 
 int main()
@@ -199,6 +245,7 @@ int main()
         loop_checks<float, std::int64_t>("f32/int64");
         error_checks();
         npy_check();
+        harness_checks();
     } catch (std::exception const& e) {
         std::cout << "FAIL exception: " << e.what() << '\n';
         return 1;

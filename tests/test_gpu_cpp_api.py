@@ -57,3 +57,44 @@ def test_driver_matches_oracle(orc, tmp_path, real):
             got = arrs[s]
             assert got.dtype == dtype and got.shape == (na, na, na)
             assert np.array_equal(got, ref), (loop, it)
+
+
+def test_benchmarks_write_google_benchmark_json(tmp_path):
+    """euler_benchmarks (the counterpart of the reference's benchmarks/ binary)
+    writes Google-Benchmark-shaped JSON that plot.py's loader reads: names
+    <Family>/<size>, per-iteration cpu_time in ns and the iteration-invariant
+    counters cells_per_second / bytes_per_second = per-iteration cells or bytes
+    divided by the CPU time, with the reference's sizes and byte formulas."""
+    import json
+
+    out = tmp_path / "bm.json"
+    r = _run(["euler_benchmarks", "--benchmark_filter=^(Godunov|TimeStep|ConsToPrim|PrimToCons)/(8|16|32)$|"
+              "^EulerSimulation/16$", "--benchmark_min_time=0.02s", f"--benchmark_out={out}",
+              "--benchmark_out_format=json"])
+    assert r.returncode == 0
+    data = json.loads(out.read_text())
+    assert data["context"]["real_type"] == "double" and data["context"]["index_type"] == "int"
+    rows = {b["name"]: b for b in data["benchmarks"]}
+    expected = {f"{f}/{n}" for f in ("Godunov", "TimeStep", "ConsToPrim", "PrimToCons") for n in (8, 16, 32)}
+    expected.add("EulerSimulation/16")
+    assert set(rows) == expected
+    s = 8
+    for name, b in rows.items():
+        family, size = name.split("/")[0], int(name.split("/")[-1])  # plot.py's parse
+        assert b["time_unit"] == "ns" and b["iterations"] >= 1 and b["cpu_time"] > 0 and b["real_time"] > 0
+        cpu_s = b["cpu_time"] * 1e-9
+        if family == "Godunov":
+            cells, nbytes = (size + 2) ** 3, 15 * s * (size + 2) ** 3
+        elif family == "TimeStep":
+            cells, nbytes = size ** 3, 5 * s * size ** 3
+        elif family in ("ConsToPrim", "PrimToCons"):
+            cells, nbytes = size ** 3, 10 * s * size ** 3
+        else:
+            cells, nbytes = (size + 2) ** 3, None
+        assert b["cells_per_second"] == pytest.approx(cells / cpu_s, rel=1e-9)
+        if nbytes is None:
+            assert "bytes_per_second" not in b
+        else:
+            assert b["bytes_per_second"] == pytest.approx(nbytes / cpu_s, rel=1e-9)
+        # the host spins while it waits, so CPU time tracks real time
+        assert 0.5 < b["cpu_time"] / b["real_time"] < 1.5, name
