@@ -142,7 +142,7 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
     eb.prim_to_cons(ex, prim, cons, eos)
     sim = eb.simulation.Simulation(ex, prim, cons, eos, mesh, 0.5)
     sim.run(4, use_graph=True)
-    ms_plain = timer.run(lambda: sim.run(nsteps, use_graph=True), 1)
+    ms_plain = statistics.median(timer.run(lambda: sim.run(nsteps, use_graph=True), 1) for _ in range(3))
     del sim, prim, cons
 
     def wall(fn):
@@ -166,10 +166,15 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
         (out[0] * out[0] * mask).sum().backward()
         return [t.grad for t in ts], tape[0].dts.clone()
 
-    fwd_only()
-    fwd_bwd()
-    ms_fwd, _ = wall(fwd_only)
-    ms_total, (grads, dts) = wall(fwd_bwd)
+    # two warm-up rounds (first-touch allocations of the caching allocator), then
+    # the median of three timed runs of each
+    for _ in range(2):
+        fwd_only()
+        fwd_bwd()
+    ms_fwd = statistics.median(wall(fwd_only)[0] for _ in range(3))
+    runs = [wall(fwd_bwd) for _ in range(3)]
+    ms_total = statistics.median(r[0] for r in runs)
+    grads, dts = runs[-1][1]
 
     def loss(rho):
         with torch.no_grad():
@@ -192,6 +197,7 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
         "plain_forward_ms": round(ms_plain, 2),
         "rollout_forward_ms": round(ms_fwd, 2),
         "forward_plus_adjoint_ms": round(ms_total, 2),
+        "timing": "median of 3 runs after 2 warm-up rounds; plain forward: graph-replayed fused steps",
         "ratio_vs_plain_forward": round(ms_total / ms_plain, 3),
         "ratio_vs_rollout_forward": round(ms_total / ms_fwd, 3),
         "fd_check": {"directions": ndirs, "eps": eps, "field": "rho (N(0,1)*bump, seeds 5678+j)",

@@ -22,6 +22,7 @@ loss costs less than a full-grid one.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import List, Optional, Sequence
 
 import torch
@@ -79,23 +80,47 @@ class Rollout:
         self._segment: List[ops.EulerPrimArrays] = []
         self.tail = 0
         self._tail_ready = False
+        self._geom_args = None
+        self._ptrs: dict = {}
+        self._cfl_arg = None
 
     # ---------------------------------------------------------------- launches
     def _geom(self):
-        n0, n1, n2 = ops.extents(self.prim0)
-        m = self.mesh
-        r = lambda x: ops._real(self.suf, x)  # noqa: E731
-        return n0, n1, n2, self.prim0.index_bits, r(self.eos.gamma), r(m.dx0), r(m.dx1), r(m.dx2)
+        # extents, index width, gamma and cell widths as C arguments, built once per forward
+        if self._geom_args is None:
+            n0, n1, n2 = ops.extents(self.prim0)
+            m = self.mesh
+            r = lambda x: ops._real(self.suf, x)  # noqa: E731
+            self._geom_args = (n0, n1, n2, self.prim0.index_bits, r(self.eos.gamma), r(m.dx0), r(m.dx1), r(m.dx2))
+        return self._geom_args
+
+    def _p(self, a: ops.Arrays):
+        """The C pointer struct of one of the rollout's states, passed by reference;
+        built once per state (the launches of a rollout reuse a few buffers, and
+        rebuilding the struct per call costs more host time than the small adjoint
+        steps take on the GPU)."""
+        hit = self._ptrs.get(id(a))
+        if hit is None or hit[0]() is not a:
+            st = ops.prim_ptrs(a) if isinstance(a, ops.EulerPrimArrays) else ops.cons_ptrs(a)
+            hit = (weakref.ref(a), st, ctypes.byref(st))
+            self._ptrs[id(a)] = hit
+        return hit[2]
+
+    def _slot_ptr(self, i: int) -> int:
+        # [0]: discarded dt, [1]: ghost-cell dt minimum
+        return self._slots.data_ptr() + i * self._slots.element_size()
+
+    def _dt_ptr(self, n: int) -> int:
+        return self.dts.data_ptr() + n * self.dts.element_size()
 
     def _ghost_refresh(self, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays):
-        call(f"eb_ghost_refresh_{self.suf}", self.ex.context, self.ex.handle, ctypes.byref(ops.cons_ptrs(cons)),
-             ctypes.byref(ops.prim_ptrs(dst)), *self._geom(), self._slots[1:].data_ptr())
+        call(f"eb_ghost_refresh_{self.suf}", self.ex.context, self.ex.handle, self._p(cons), self._p(dst),
+             *self._geom(), self._slot_ptr(1))
 
     def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
                dt_out: int, region=None):
-        args = (self.ex.context, self.ex.handle, ctypes.byref(ops.prim_ptrs(src)), ctypes.byref(ops.cons_ptrs(cons)),
-                ctypes.byref(ops.prim_ptrs(dst)), *self._geom(), ops._real(self.suf, self.cfl), dt_in,
-                self._slots[1:].data_ptr(), dt_out, None)
+        args = (self.ex.context, self.ex.handle, self._p(src), self._p(cons), self._p(dst), *self._geom(),
+                self._cfl_arg, dt_in, self._slot_ptr(1), dt_out, None)
         if region is None:
             call(f"eb_step_fused_{self.suf}", *args)
         else:
@@ -111,6 +136,9 @@ class Rollout:
         self.suf = ops._validate(prim0)
         if min(ops.extents(prim0)) < 3:
             raise ValueError("a rollout needs at least one interior cell")
+        self._geom_args = None
+        self._ptrs = {}
+        self._cfl_arg = ops._real(self.suf, self.cfl)
         dtype, dev = prim0.d.dtype, prim0.d.device
         self.prim0 = _clone_prim(prim0)
         self._slots = torch.zeros(2, dtype=dtype, device=dev)
@@ -125,6 +153,7 @@ class Rollout:
             dt = torch.empty(self.nsteps + 1, dtype=dtype, device=dev)
             ops.time_step_async(self.ex, self.prim0, self.eos, self.mesh, out=dt[0:1])
         self.dts = dt[:self.nsteps]
+        dt_base, esz = dt.data_ptr(), dt.element_size()
         # The last segment's states are written straight into the segment buffers,
         # so the backward starts without recomputing it (and needs no checkpoint for it).
         self.tail = ((self.nsteps - 1) // self.k) * self.k if keep else self.nsteps
@@ -141,8 +170,8 @@ class Rollout:
                 nxt = self._segment[m]
             else:
                 nxt = buf[1] if cur is buf[0] else buf[0]
-            out = self._slots[0:].data_ptr() if frozen_dts is not None else dt[n + 1:].data_ptr()
-            self._fused(cur, cons, nxt, dt[n:].data_ptr(), out)
+            out = self._slot_ptr(0) if frozen_dts is not None else dt_base + (n + 1) * esz
+            self._fused(cur, cons, nxt, dt_base + n * esz, out)
             if n == 0:
                 # later steps read ghost primitives round-tripped through cons_to_prim
                 self._ghost_refresh(cons, buf[0])
@@ -182,7 +211,7 @@ class Rollout:
             prims = [self._segment[0]]
         for n in range(start, stop - 1):
             nxt = self._segment[n - start + 1]
-            self._fused(prims[-1], cons, nxt, self.dts[n:].data_ptr(), self._slots[0:].data_ptr(),
+            self._fused(prims[-1], cons, nxt, self._dt_ptr(n), self._slot_ptr(0),
                         None if needed is None else needed(n + 1))
             prims.append(nxt)
         return prims
@@ -214,9 +243,8 @@ class Rollout:
             box = [lo[2], hi[2], lo[1], hi[1], lo[0], hi[0]]
         mu = _zeros_cons_like(self.prim0)
         mu_next = _zeros_cons_like(self.prim0)
-        call(f"eb_cons_to_prim_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(self.prim_final)),
-             ctypes.byref(ops.prim_ptrs(gP)), ctypes.byref(ops.cons_ptrs(mu)), n0, n1, n2, bits,
-             r(self.eos.gamma))
+        call(f"eb_cons_to_prim_vjp_{suf}", h, self._p(self.prim_final), ctypes.byref(ops.prim_ptrs(gP)),
+             self._p(mu), n0, n1, n2, bits, r(self.eos.gamma))
         gprim0 = _zeros_prim_like(self.prim0)
         step = 0
 
@@ -243,16 +271,14 @@ class Rollout:
                 prims = self._recompute(start, stop, needed)
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
-                dt = self.dts[n:n + 1].data_ptr()
+                dt = self._dt_ptr(n)
                 step += 1
                 if n == 0:
-                    call(f"eb_godunov_vjp_first_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
-                         ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.prim_ptrs(gprim0)), *geom, dt,
-                         r(self.cfl), region())
+                    call(f"eb_godunov_vjp_first_{suf}", h, self._p(P), self._p(mu), self._p(gprim0), *geom, dt,
+                         self._cfl_arg, region())
                 else:
-                    call(f"eb_godunov_vjp_{suf}", h, ctypes.byref(ops.prim_ptrs(P)),
-                         ctypes.byref(ops.cons_ptrs(mu)), ctypes.byref(ops.cons_ptrs(mu_next)), *geom, dt,
-                         r(self.cfl), region())
+                    call(f"eb_godunov_vjp_{suf}", h, self._p(P), self._p(mu), self._p(mu_next), *geom, dt,
+                         self._cfl_arg, region())
                     mu, mu_next = mu_next, mu
         return gprim0
 

@@ -123,7 +123,9 @@ __device__ __forceinline__ void hllc_vjp(
     Prim<T> const& q = left ? qL : qR;
     T const S = left ? SL : SR;
     T const un = left ? unL : unR;
-    T const etot = total_energy(q, gm1);
+    // 1/(gamma-1) is loop-invariant in the kernels, so this is hoisted
+    T const igm1 = fast_rcp(gm1);
+    T const etot = q.p * igm1 + ke_prim(q);
     bool const sup = SL * SR > T(0);
     T const uo = sup ? un : us;
     T const po = sup ? q.p : ps;
@@ -195,8 +197,8 @@ __device__ __forceinline__ void hllc_vjp(
     // etot = p/gm1 + d*|u|^2/2 of the upwind state
     {
         T const u2 = (q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2);
-        gq.p += g_etot / gm1;
-        gq.d += g_etot * u2 / T(2);
+        gq.p += g_etot * igm1;
+        gq.d += g_etot * u2 * T(0.5);
         gq.u0 += g_etot * q.d * q.u0;
         gq.u1 += g_etot * q.d * q.u1;
         gq.u2 += g_etot * q.d * q.u2;
@@ -303,8 +305,8 @@ __device__ __forceinline__ void hllc_vjp(
 template <class T>
 __device__ __forceinline__ Cons<T> c2p_vjp(Prim<T> const& q, PrimGrad<T> const& g, T gm1)
 {
-    T const inv = T(1) / q.d;
-    T const ke = q.d * ((q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2)) / T(2);
+    T const inv = fast_rcp(q.d);
+    T const ke = q.d * ((q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2)) * T(0.5);
     T const g_ke = -gm1 * g.p;
     Cons<T> r;
     r.d = g.d - g_ke * ke * inv - (g.u0 * q.u0 + g.u1 * q.u1 + g.u2 * q.u2) * inv;
@@ -321,8 +323,8 @@ __device__ __forceinline__ PrimGrad<T> p2c_vjp(Prim<T> const& q, Cons<T> const& 
 {
     T const u2 = (q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2);
     PrimGrad<T> r;
-    r.d = g.d + g.e * u2 / T(2) + g.m0 * q.u0 + g.m1 * q.u1 + g.m2 * q.u2;
-    r.p = g.e / gm1;
+    r.d = g.d + g.e * u2 * T(0.5) + g.m0 * q.u0 + g.m1 * q.u1 + g.m2 * q.u2;
+    r.p = g.e * fast_rcp(gm1);
     r.u0 = g.e * q.d * q.u0 + g.m0 * q.d;
     r.u1 = g.e * q.d * q.u1 + g.m1 * q.d;
     r.u2 = g.e * q.d * q.u2 + g.m2 * q.d;
