@@ -83,3 +83,32 @@ def test_cpp_headers_compile_standalone(tmp_path):
                             "-D__HIP_PLATFORM_AMD__", f"-I{inc}", "-I/opt/rocm/include", str(src)],
                            capture_output=True, text=True)
         assert r.returncode == 0, (h.name, r.stderr[-2000:])
+
+
+def test_argument_checks_without_gpu():
+    """Host-side argument validation answers before any device work: bad
+    configurations, a negative fill count, a null fill target, an unknown real
+    width for the tile-height query."""
+    lib = ctypes.CDLL(str(LIB))
+    lib.eb_last_error.restype = ctypes.c_char_p
+    EB_OK = 0
+    assert lib.eb_set_godunov_config(1, 5, 0) != EB_OK
+    assert lib.eb_set_godunov_config(1, 4, -1) != EB_OK
+    assert lib.eb_set_godunov_config(4, 0, 0) != EB_OK
+    v, by, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.eb_get_godunov_config(ctypes.byref(v), ctypes.byref(by), ctypes.byref(kc)) == EB_OK
+    assert (v.value, by.value, kc.value) == (1, 0, 0)
+    h = ctypes.c_int()
+    assert lib.eb_get_godunov_tile_height(8, 0, ctypes.byref(h)) == EB_OK and h.value == 4
+    assert lib.eb_get_godunov_tile_height(8, 1, ctypes.byref(h)) == EB_OK and h.value == 8
+    assert lib.eb_get_godunov_tile_height(4, 0, ctypes.byref(h)) == EB_OK and h.value == 8
+    assert lib.eb_get_godunov_tile_height(2, 0, ctypes.byref(h)) != EB_OK
+    assert lib.eb_set_godunov_config(1, 16, 8) == EB_OK
+    assert lib.eb_get_godunov_tile_height(8, 0, ctypes.byref(h)) == EB_OK and h.value == 16
+    assert lib.eb_set_godunov_config(1, 0, 0) == EB_OK
+    lib.eb_fill_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double]
+    assert lib.eb_fill_f64(None, None, -1, 0.0) != EB_OK
+    assert b"negative" in lib.eb_last_error()
+    assert lib.eb_fill_f64(None, None, 10, 0.0) != EB_OK
+    assert b"null" in lib.eb_last_error()
+    assert lib.eb_fill_f64(None, None, 0, 0.0) == EB_OK  # nothing to do

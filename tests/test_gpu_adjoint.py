@@ -386,3 +386,49 @@ def test_rollout_without_grad_keeps_nothing_and_matches(eb, ex):
         assert torch.equal(a.detach(), b)
     with pytest.raises(RuntimeError):
         tape2[0].backward([torch.zeros_like(t) for t in out2])
+
+
+@pytest.mark.parametrize("box", [(5, 4, 1, 30, 1, 30),       # empty (lo > hi): no work
+                                 (40, 60, 1, 5, 1, 5),       # outside the interior: no work
+                                 (-10, 3, 18, 99, -5, 2)])   # clipped to the interior
+def test_step_fused_region_edges(eb, ex, box):
+    """Boxes that are empty or outside the interior launch nothing and leave every
+    array as it was; a box reaching past the interior is clipped to it."""
+    from benchmark_euler_kernels_amd._lib import call
+
+    ops = eb.operators
+    shape = (24, 20, 22)
+    dx = (1 / 22, 1 / 18, 1 / 20)
+    P0 = _smooth_state(shape, 8)
+    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)])
+    cons0 = eb.EulerConsArrays(*[torch.empty_like(t) for t in prim.tensors()])
+    ops.prim_to_cons(ex, prim, cons0, eb.PerfectGas(G))
+    dt_in = torch.tensor([1e-3], dtype=torch.float64, device="cuda")
+    big = torch.tensor([1e300], dtype=torch.float64, device="cuda")
+    results = []
+    for region in (None, (ctypes.c_int64 * 6)(*box)):
+        cons = eb.EulerConsArrays(*[t.clone() for t in cons0.tensors()])
+        out = eb.EulerPrimArrays(*[torch.full_like(t, 3.0) for t in prim.tensors()])
+        dt_out = torch.full((1,), -1.0, dtype=torch.float64, device="cuda")
+        args = (ex.context, ex.handle, ctypes.byref(ops.prim_ptrs(prim)), ctypes.byref(ops.cons_ptrs(cons)),
+                ctypes.byref(ops.prim_ptrs(out)), *shape, 64, ctypes.c_double(G), *map(ctypes.c_double, dx),
+                ctypes.c_double(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr(), None)
+        if region is None:
+            call("eb_step_fused_f64", *args)
+        else:
+            call("eb_step_fused_region_f64", *args, region)
+        torch.cuda.synchronize()
+        results.append((torch.stack(cons.tensors()), torch.stack(out.tensors()), dt_out.item()))
+    (c_full, p_full, _), (c_box, p_box, dt_box) = results
+    lo = [max(box[2 * d], 1) for d in range(3)]
+    hi = [min(box[2 * d + 1], shape[d] - 2) for d in range(3)]
+    inside = torch.zeros(shape[2], shape[1], shape[0], dtype=torch.bool, device="cuda")
+    if all(lo[d] <= hi[d] for d in range(3)):
+        inside[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+        assert torch.equal(p_box[:, inside], p_full[:, inside])
+        assert torch.equal(c_box[:, inside], c_full[:, inside])
+        assert dt_box > 0
+    else:
+        assert dt_box == -1.0  # nothing launched, dt_out untouched
+    assert torch.equal(p_box[:, ~inside], torch.full_like(p_box[:, ~inside], 3.0))
+    assert torch.equal(c_box[:, ~inside], torch.stack(cons0.tensors())[:, ~inside])
