@@ -360,7 +360,8 @@ def kernel_name(eb) -> str:
     """The FP64 flux kernel the current configuration launches (int64 indices)."""
     variant = eb.get_godunov_config()[0]
     by = eb.godunov_tile_height(fused=False)
-    kind = {0: "k_godunov_faces", 1: "k_godunov_tiled", 2: "k_godunov_stream", 3: "k_godunov_linear"}[variant]
+    kind = {0: "k_godunov_faces", 1: "k_godunov_tiled", 2: "k_godunov_stream", 3: "k_godunov_linear",
+            4: "k_godunov_pair"}[variant]
     return f"eb::{kind}<double, long, {by}, false>"
 
 

@@ -28,26 +28,31 @@ thread_local std::string g_last_error;
 // Godunov launch configuration (process-wide): kernel variant, tile height, planes
 // per block. Variant 0 = face-parallel waves, 1 = cell-parallel waves (default,
 // fastest measured at 512^3 on MI355X), 2 = streamed with a producer wave.
-std::atomic<int> g_variant {1};
+std::atomic<int> g_variant {4};
 std::atomic<int> g_tile_by {0};     // 0: chosen per kernel by auto_by
 std::atomic<int> g_tile_kchunk {0}; // 0: sized per launch by auto_kchunk
 
 bool config_supported(int variant, int by)
 {
-    return (variant >= 0 && variant <= 3 && by == 0) || (variant == 0 && (by == 2 || by == 4))
+    return (variant >= 0 && variant <= 4 && by == 0) || (variant == 0 && (by == 2 || by == 4))
+           || (variant == 4 && (by == 8 || by == 16))
            || (variant == 1 && (by == 4 || by == 8 || by == 16)) || (variant == 2 && (by == 4 || by == 8))
            || (variant == 3 && (by == 4 || by == 8));
 }
 
 // Tile height when the configuration leaves it open (by = 0), from MI355X timings
-// at 512^3. The FP64 flux kernel of the 64 x by tiles needs ~158 VGPRs: three
-// waves per SIMD, which 4-wave blocks (three per CU) can use and 8-wave blocks
-// (one per CU) cannot, so it takes by = 4. Its fused form (~172 VGPRs, two waves
-// per SIMD) and the FP32 kernels (~94-108 VGPRs) run best with by = 8.
+// at 512^3. Paired rows (variant 4, the default): 64 x 16 tiles, eight waves of two
+// rows each, for the flux kernel and the fused step in FP64 and FP32. One row per
+// lane (variant 1): the FP64 flux kernel needs ~148 VGPRs, three waves per SIMD,
+// which 4-wave blocks (three per CU) can use and 8-wave blocks (one per CU) cannot,
+// so it takes by = 4; its fused form and the FP32 kernels run best with by = 8.
 int auto_by(int variant, std::size_t real_bytes, bool fused)
 {
     if (variant == 0) {
         return 4;
+    }
+    if (variant == 4) {
+        return 16;
     }
     if (variant == 1 && real_bytes == 8 && !fused) {
         return 4;
@@ -250,6 +255,11 @@ void launch_tiled(
                            static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
                            tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
                            sink);
+    } else if constexpr (Variant == 4) {
+        hipLaunchKernelGGL((eb::k_godunov_pair<T, I, BY, Fused>), dim3(nblk), dim3(64, BY / 2), 0, s, pc, cu, pout,
+                           static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
+                           tiles_z, gamma, static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2,
+                           sink);
     } else if constexpr (Variant == 0) {
         hipLaunchKernelGGL((eb::k_godunov_faces<T, I, BY, Fused>), dim3(nblk), dim3(64, 3 * BY), 0, s, pc, cu, pout,
                            static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, tiles_x, tiles_y,
@@ -354,7 +364,11 @@ int dispatch_tiled(
     if (variant == 3 && n0 > 64 * by) {
         return fail(EB_ERR_UNSUPPORTED, "linear tiles need rows of at most 64*by cells (n0 <= 64*by)");
     }
-    if (variant == 3 && by == 4) {
+    if (variant == 4 && by == 8) {
+        EB_LAUNCH(4, 8);
+    } else if (variant == 4 && by == 16) {
+        EB_LAUNCH(4, 16);
+    } else if (variant == 3 && by == 4) {
         EB_LAUNCH(3, 4);
     } else if (variant == 3 && by == 8) {
         EB_LAUNCH(3, 8);
@@ -617,8 +631,8 @@ int eb_set_godunov_config(int variant, int by, int kchunk)
     }
     if (!config_supported(variant, by)) {
         return fail(EB_ERR_UNSUPPORTED,
-                    "unsupported (variant, tile height): (0-3, 0 = automatic), (0, 2|4), (1, 4|8|16), (2, 4|8), "
-                    "(3, 4|8)");
+                    "unsupported (variant, tile height): (0-4, 0 = automatic), (0, 2|4), (1, 4|8|16), (2, 4|8), "
+                    "(3, 4|8), (4, 8|16)");
     }
     g_variant.store(variant);
     g_tile_by.store(by);

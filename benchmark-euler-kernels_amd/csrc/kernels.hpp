@@ -284,6 +284,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk)
     return (b & 7) * (nblk >> 3) + (b >> 3);
 }
 
+// Hides how a wave-uniform value was computed from the optimiser (an empty asm
+// that keeps it in scalar registers). Used on the plane index of a march so that
+// loop strength reduction does not turn the scalar plane base plus 32-bit lane
+// offset into per-lane 64-bit pointers carried across iterations (two VGPRs per
+// field and cell).
+template <class I>
+__device__ __forceinline__ void uniform_opaque(I& v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(v));
+#endif
+}
+
 // Plane-relative addressing: a wave-uniform plane base pointer (scalar registers,
 // index type I, so 64-bit extents are honoured) plus a per-lane 32-bit byte offset
 // inside the plane. This is the saddr + voffset form of global loads/stores: one
@@ -524,7 +537,8 @@ __device__ __forceinline__ T godunov_tile(
 #endif
 
     for (I k = kbeg; k < kend; ++k) {
-        I const plane = s2 * k;
+        I plane = s2 * k;
+        uniform_opaque(plane);
         q_next = ld_prim(prim, plane + s2, own);
 #if EB_HALO_PREFETCH
         // halo of the next plane, one plane ahead like the own cell
@@ -708,6 +722,304 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_box(
     T const bm = block_min<T, BY>(local_min);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+    }
+}
+
+// Paired-row Godunov update (2.5-D march along k), configuration variant 4.
+//
+// A block of BY/2 waves owns a 64 x BY column of interior cells; each lane owns two
+// cells of one column, rows 2w and 2w+1 of the tile (cells A and B of wave w). The
+// face between A and B is solved once in registers and serves both cells; the face
+// below A goes through LDS to the wave below (whose B it tops); x faces come from
+// the next lane by shuffle; z faces are carried to the next plane. Compared with one
+// cell per lane, a tile twice as tall costs the same number of waves: half the halo
+// rows and half the tile-edge face work per cell, two independent cells per lane,
+// and two barriers per plane for 64*BY cells. Same per-cell sums as the reference
+// (x, y, z order, godunov.hpp:48-82), so bitwise equal to the other variants.
+template <class T, class I, int BY, bool Fused>
+__device__ __forceinline__ T godunov_tile_pair(
+        PrimPtrs<T const> const& prim,
+        ConsPtrs<T> const& cons,
+        PrimPtrs<T> const& prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int bx,
+        int by,
+        int bz,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        T dtodv,
+        T invdx0,
+        T invdx1,
+        T invdx2)
+{
+    static_assert(BY % 2 == 0, "two rows per wave");
+    static_assert(2 * 64 + 2 * BY <= 32 * BY, "one halo cell per thread at most (BY >= 8)");
+    constexpr int BX = 64;
+    constexpr int NW = BY / 2;
+    constexpr int LX = BX + 2;
+    constexpr int LY = BY + 2;
+    constexpr int NHALO = 2 * BX + 2 * BY;
+    constexpr int NF = 7; // staged per cell: d, p, u0, u1, u2, c, E
+    __shared__ T sq[NF][LY][LX];
+    __shared__ T sfy[5][NW + 1][BX]; // [w]: face between staged rows 2w and 2w+1 (below wave w's A)
+    __shared__ T sfx[5][BY];         // x faces on the tile's high-i edge, per row
+
+    int const tx = threadIdx.x;
+    int const ty = threadIdx.y;
+    int const lid = tx + BX * ty;
+    int const rA = 2 * ty + 1; // staged rows of A and B
+    int const rB = 2 * ty + 2;
+
+    I const i0 = 1 + static_cast<I>(bx) * BX;
+    I const j0 = 1 + static_cast<I>(by) * BY;
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    I const s2 = n0 * n1;
+
+    I const i = i0 + tx;
+    I const jA = j0 + 2 * ty;
+    I const jB = jA + 1;
+    bool const validA = (i <= n0 - 2) && (jA <= n1 - 2);
+    bool const validB = (i <= n0 - 2) && (jB <= n1 - 2);
+    I const ic = (i < n0 - 1) ? i : n0 - 1;
+    I const jAc = (jA < n1 - 1) ? jA : n1 - 1;
+    I const jBc = (jB < n1 - 1) ? jB : n1 - 1;
+    uint32_t const offA = static_cast<uint32_t>(ic + n0 * jAc) * static_cast<uint32_t>(sizeof(T));
+    uint32_t const offB = static_cast<uint32_t>(ic + n0 * jBc) * static_cast<uint32_t>(sizeof(T));
+
+    int hrow = 0;
+    int hcol = 0;
+    uint32_t halo = 0;
+    bool const has_halo = lid < NHALO;
+    if (has_halo) {
+        if (lid < BX) {
+            hrow = 0;
+            hcol = lid + 1;
+        } else if (lid < 2 * BX) {
+            hrow = BY + 1;
+            hcol = lid - BX + 1;
+        } else if (lid < 2 * BX + BY) {
+            hrow = lid - 2 * BX + 1;
+            hcol = 0;
+        } else {
+            hrow = lid - 2 * BX - BY + 1;
+            hcol = LX - 1;
+        }
+        I hi = i0 - 1 + hcol;
+        I hj = j0 - 1 + hrow;
+        hi = (hi < n0 - 1) ? hi : n0 - 1;
+        hj = (hj < n1 - 1) ? hj : n1 - 1;
+        halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
+    }
+
+    T local_min = Limits<T>::max();
+
+    auto stage = [&](int r, int c, Prim<T> const& q, T cs, T e) {
+        sq[0][r][c] = q.d;
+        sq[1][r][c] = q.p;
+        sq[2][r][c] = q.u0;
+        sq[3][r][c] = q.u1;
+        sq[4][r][c] = q.u2;
+        sq[5][r][c] = cs;
+        sq[6][r][c] = e;
+    };
+    auto cell = [&](int r, int c) { return Prim<T> {sq[0][r][c], sq[1][r][c], sq[2][r][c], sq[3][r][c], sq[4][r][c]}; };
+    auto put = [](T (&dst)[5], Flux<T> const& f) {
+        dst[0] = f.d;
+        dst[1] = f.e;
+        dst[2] = f.m0;
+        dst[3] = f.m1;
+        dst[4] = f.m2;
+    };
+    auto shfl_flux = [](Flux<T> const& f) {
+        return Flux<T> {__shfl_down(f.d, 1, 64), __shfl_down(f.e, 1, 64), __shfl_down(f.m0, 1, 64),
+                        __shfl_down(f.m1, 1, 64), __shfl_down(f.m2, 1, 64)};
+    };
+
+    // Prologue: z faces below the first plane; own cells of plane kbeg in registers.
+    Prim<T> qA = ld_prim(prim, s2 * kbeg, offA);
+    Prim<T> qB = ld_prim(prim, s2 * kbeg, offB);
+    T cA = sound_speed(gamma, qA.d, qA.p);
+    T cB = sound_speed(gamma, qB.d, qB.p);
+    T eA = total_energy(qA, gm1);
+    T eB = total_energy(qB, gm1);
+    Flux<T> fzA_lo;
+    Flux<T> fzB_lo;
+    {
+        Prim<T> const m = ld_prim(prim, s2 * (kbeg - 1), offA);
+        fzA_lo = hllc_flux<2>(m, sound_speed(gamma, m.d, m.p), total_energy(m, gm1), qA, cA, eA);
+    }
+    {
+        Prim<T> const m = ld_prim(prim, s2 * (kbeg - 1), offB);
+        fzB_lo = hllc_flux<2>(m, sound_speed(gamma, m.d, m.p), total_energy(m, gm1), qB, cB, eB);
+    }
+    Prim<T> h_cur {};
+    if (has_halo) {
+        h_cur = ld_prim(prim, s2 * kbeg, halo);
+    }
+
+    for (I k = kbeg; k < kend; ++k) {
+        I plane = s2 * k;
+        uniform_opaque(plane);
+        Prim<T> const qA_next = ld_prim(prim, plane + s2, offA);
+        Prim<T> const qB_next = ld_prim(prim, plane + s2, offB);
+        Prim<T> h_next {};
+        if (has_halo) {
+            h_next = ld_prim(prim, plane + s2, halo);
+        }
+
+        // A. stage plane k.
+        stage(rA, tx + 1, qA, cA, eA);
+        stage(rB, tx + 1, qB, cB, eB);
+        if (has_halo) {
+            stage(hrow, hcol, h_cur, sound_speed(gamma, h_cur.d, h_cur.p), total_energy(h_cur, gm1));
+        }
+        lds_barrier();
+
+        // B. faces: y below A (to LDS), y between A and B (registers), the tile's
+        // high-side y and x faces, z of A and B, then x of A and B (shuffled at once).
+        // The own cells are read back from LDS, so no copy of them stays live here.
+        {
+            Flux<T> const f = hllc_flux<1>(cell(rA - 1, tx + 1), sq[5][rA - 1][tx + 1], sq[6][rA - 1][tx + 1],
+                                           cell(rA, tx + 1), sq[5][rA][tx + 1], sq[6][rA][tx + 1]);
+            T tmp[5];
+            put(tmp, f);
+            for (int c = 0; c < 5; ++c) {
+                sfy[c][ty][tx] = tmp[c];
+            }
+        }
+        Flux<T> const fAB = hllc_flux<1>(cell(rA, tx + 1), sq[5][rA][tx + 1], sq[6][rA][tx + 1], cell(rB, tx + 1),
+                                         sq[5][rB][tx + 1], sq[6][rB][tx + 1]);
+        if (ty == NW - 1) {
+            Flux<T> const f = hllc_flux<1>(cell(BY, tx + 1), sq[5][BY][tx + 1], sq[6][BY][tx + 1],
+                                           cell(BY + 1, tx + 1), sq[5][BY + 1][tx + 1], sq[6][BY + 1][tx + 1]);
+            T tmp[5];
+            put(tmp, f);
+            for (int c = 0; c < 5; ++c) {
+                sfy[c][NW][tx] = tmp[c];
+            }
+        }
+        if (ty == (NW > 1 ? NW - 2 : 0) && tx < BY) {
+            int const r = tx + 1;
+            Flux<T> const f = hllc_flux<0>(cell(r, BX), sq[5][r][BX], sq[6][r][BX], cell(r, BX + 1),
+                                           sq[5][r][BX + 1], sq[6][r][BX + 1]);
+            T tmp[5];
+            put(tmp, f);
+            for (int c = 0; c < 5; ++c) {
+                sfx[c][tx] = tmp[c];
+            }
+        }
+        T const cA_next = sound_speed(gamma, qA_next.d, qA_next.p);
+        T const eA_next = total_energy(qA_next, gm1);
+        Flux<T> const fzA_hi
+                = hllc_flux<2>(cell(rA, tx + 1), sq[5][rA][tx + 1], sq[6][rA][tx + 1], qA_next, cA_next, eA_next);
+        T const cB_next = sound_speed(gamma, qB_next.d, qB_next.p);
+        T const eB_next = total_energy(qB_next, gm1);
+        Flux<T> const fzB_hi
+                = hllc_flux<2>(cell(rB, tx + 1), sq[5][rB][tx + 1], sq[6][rB][tx + 1], qB_next, cB_next, eB_next);
+        Flux<T> const fxA_lo = hllc_flux<0>(cell(rA, tx), sq[5][rA][tx], sq[6][rA][tx], cell(rA, tx + 1),
+                                            sq[5][rA][tx + 1], sq[6][rA][tx + 1]);
+        Flux<T> fxA_hi = shfl_flux(fxA_lo);
+        Flux<T> const fxB_lo = hllc_flux<0>(cell(rB, tx), sq[5][rB][tx], sq[6][rB][tx], cell(rB, tx + 1),
+                                            sq[5][rB][tx + 1], sq[6][rB][tx + 1]);
+        Flux<T> fxB_hi = shfl_flux(fxB_lo);
+        Cons<T> uA = ld_cons(cons, plane, offA);
+        Cons<T> uB = ld_cons(cons, plane, offB);
+        lds_barrier();
+
+        // C. combine (x, y, z in the reference's order) and update both cells.
+        if (tx == BX - 1) {
+            fxA_hi = Flux<T> {sfx[0][rA - 1], sfx[1][rA - 1], sfx[2][rA - 1], sfx[3][rA - 1], sfx[4][rA - 1]};
+            fxB_hi = Flux<T> {sfx[0][rB - 1], sfx[1][rB - 1], sfx[2][rB - 1], sfx[3][rB - 1], sfx[4][rB - 1]};
+        }
+        Flux<T> const fyA_lo {sfy[0][ty][tx], sfy[1][ty][tx], sfy[2][ty][tx], sfy[3][ty][tx], sfy[4][ty][tx]};
+        Flux<T> const fyB_hi {sfy[0][ty + 1][tx], sfy[1][ty + 1][tx], sfy[2][ty + 1][tx], sfy[3][ty + 1][tx],
+                              sfy[4][ty + 1][tx]};
+        Flux<T> accA {T(0), T(0), T(0), T(0), T(0)};
+        accumulate(accA, ds0, fxA_hi, fxA_lo);
+        accumulate(accA, ds1, fAB, fyA_lo);
+        accumulate(accA, ds2, fzA_hi, fzA_lo);
+        Flux<T> accB {T(0), T(0), T(0), T(0), T(0)};
+        accumulate(accB, ds0, fxB_hi, fxB_lo);
+        accumulate(accB, ds1, fyB_hi, fAB);
+        accumulate(accB, ds2, fzB_hi, fzB_lo);
+        apply_update(uA, dtodv, accA);
+        apply_update(uB, dtodv, accB);
+        if (validA) {
+            st_cons(cons, plane, offA, uA);
+            if constexpr (Fused) {
+                Prim<T> const q1 = cons_to_prim_pt(uA, gm1);
+                st_prim(prim_out, plane, offA, q1);
+                T const c1 = sound_speed(gamma, q1.d, q1.p);
+                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+            }
+        }
+        if (validB) {
+            st_cons(cons, plane, offB, uB);
+            if constexpr (Fused) {
+                Prim<T> const q1 = cons_to_prim_pt(uB, gm1);
+                st_prim(prim_out, plane, offB, q1);
+                T const c1 = sound_speed(gamma, q1.d, q1.p);
+                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+            }
+        }
+
+        // Advance the march.
+        fzA_lo = fzA_hi;
+        fzB_lo = fzB_hi;
+        qA = qA_next;
+        qB = qB_next;
+        cA = cA_next;
+        cB = cB_next;
+        eA = eA_next;
+        eB = eB_next;
+        h_cur = h_next;
+    }
+    return local_min;
+}
+
+template <class T, class I, int BY, bool Fused>
+__global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) void k_godunov_pair(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    int const bx = t % tiles_x;
+    int const by = (t / tiles_x) % tiles_y;
+    int const bz = t / (tiles_x * tiles_y);
+    T const local_min = godunov_tile_pair<T, I, BY, Fused>(prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz,
+                                                            gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta), invdx0,
+                                                            invdx1, invdx2);
+    if constexpr (Fused) {
+        T const bm = block_min<T, BY / 2>(local_min);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+        }
     }
 }
 

@@ -370,9 +370,9 @@ def boundary_flux(exec_space: ExecSpace, prim_arrays: EulerPrimArrays, eos: Perf
 
 def set_godunov_config(variant: int, by: int, kchunk: int) -> None:
     """Godunov kernel variant (0 face-parallel, 1 cell-parallel, 2 streamed,
-    3 linear tiles), tile height (0: chosen per kernel) and planes marched per
-    block (0: sized per launch from the grid). Default (1, 0, 0). Results do not
-    depend on it."""
+    3 linear tiles, 4 cell-parallel with two rows per wave), tile height (0:
+    chosen per kernel) and planes marched per block (0: sized per launch from the
+    grid). Default (4, 0, 0). Results do not depend on it."""
     call("eb_set_godunov_config", variant, by, kchunk)
 
 

@@ -56,17 +56,19 @@ const char* eb_last_error(void);
 
 /* Godunov kernel configuration, process-wide (used by eb_godunov*, eb_step_fused*):
  *   variant 0: face-parallel waves, tile height `by` in {2, 4};
- *   variant 1: cell-parallel waves, `by` in {4, 8, 16} (default);
+ *   variant 1: cell-parallel waves, one row of cells per wave, `by` in {4, 8, 16};
  *   variant 2: streamed (producer wave + global->LDS transfers), `by` in {4, 8};
  *              needs 16-byte aligned primitive fields (else EB_ERR_UNSUPPORTED);
  *   variant 3: linear tiles of 64*by consecutive cells of a plane (rows wrap, no
  *              lane lost to ragged x extents), `by` in {4, 8}; needs n0 <= 64*by
  *              (else EB_ERR_UNSUPPORTED);
- *   `by` 0 (any variant): the tile height is chosen per kernel (variant 1: 4 for
- *   the FP64 flux kernel, 8 for its fused form and for FP32);
+ *   variant 4: cell-parallel waves, two rows of cells per wave (the face between
+ *              them solved once in registers), `by` in {8, 16} rows (default);
+ *   `by` 0 (any variant): the tile height is chosen per kernel (variant 4: 16;
+ *   variant 1: 4 for the FP64 flux kernel, 8 for its fused form and for FP32);
  *   `kchunk` >= 1 planes marched per block along k, or 0: chosen per launch from
  *   the grid (long chunks, shortened until the launch has ~512 blocks).
- *   Defaults (1, 0, 0).
+ *   Defaults (4, 0, 0).
  * All configurations give bitwise identical results. */
 int eb_set_godunov_config(int variant, int by, int kchunk);
 int eb_get_godunov_config(int* variant, int* by, int* kchunk);

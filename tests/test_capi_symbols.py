@@ -94,18 +94,20 @@ def test_argument_checks_without_gpu():
     EB_OK = 0
     assert lib.eb_set_godunov_config(1, 5, 0) != EB_OK
     assert lib.eb_set_godunov_config(1, 4, -1) != EB_OK
-    assert lib.eb_set_godunov_config(4, 0, 0) != EB_OK
+    assert lib.eb_set_godunov_config(5, 0, 0) != EB_OK
     v, by, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     assert lib.eb_get_godunov_config(ctypes.byref(v), ctypes.byref(by), ctypes.byref(kc)) == EB_OK
-    assert (v.value, by.value, kc.value) == (1, 0, 0)
+    assert (v.value, by.value, kc.value) == (4, 0, 0)
     h = ctypes.c_int()
+    assert lib.eb_get_godunov_tile_height(8, 0, ctypes.byref(h)) == EB_OK and h.value == 16
+    assert lib.eb_get_godunov_tile_height(2, 0, ctypes.byref(h)) != EB_OK
+    assert lib.eb_set_godunov_config(1, 0, 0) == EB_OK
     assert lib.eb_get_godunov_tile_height(8, 0, ctypes.byref(h)) == EB_OK and h.value == 4
     assert lib.eb_get_godunov_tile_height(8, 1, ctypes.byref(h)) == EB_OK and h.value == 8
     assert lib.eb_get_godunov_tile_height(4, 0, ctypes.byref(h)) == EB_OK and h.value == 8
-    assert lib.eb_get_godunov_tile_height(2, 0, ctypes.byref(h)) != EB_OK
     assert lib.eb_set_godunov_config(1, 16, 8) == EB_OK
     assert lib.eb_get_godunov_tile_height(8, 0, ctypes.byref(h)) == EB_OK and h.value == 16
-    assert lib.eb_set_godunov_config(1, 0, 0) == EB_OK
+    assert lib.eb_set_godunov_config(4, 0, 0) == EB_OK
     lib.eb_fill_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double]
     assert lib.eb_fill_f64(None, None, -1, 0.0) != EB_OK
     assert b"negative" in lib.eb_last_error()

@@ -91,7 +91,7 @@ def test_time_step_uniform_closed_form(eb, ex):
 @pytest.mark.parametrize("index_bits", [32, 64])
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("tile", [(0, 4, 64), (0, 2, 3), (1, 8, 64), (1, 4, 3), (1, 16, 1), (2, 4, 64), (2, 8, 3),
-                                  (2, 4, 1), (3, 4, 64), (3, 8, 3), (3, 4, 1), (1, 0, 0), (3, 0, 5)])
+                                  (2, 4, 1), (3, 4, 64), (3, 8, 3), (3, 4, 1), (1, 0, 0), (3, 0, 5), (4, 8, 64), (4, 8, 3), (4, 16, 1), (4, 16, 5), (1, 4, 64), (4, 0, 0)])
 def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
     print(f"godunov dtype={dtype.__name__} bits={index_bits} shape={shape} tile={tile}", flush=True)
     gamma = 1.4
@@ -110,7 +110,7 @@ def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
         eb.godunov(ex, P, C, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx), eb.hllc(), float(dt))
         got = to_host(C)
     finally:
-        eb.set_godunov_config(1, 0, 0)
+        eb.set_godunov_config(4, 0, 0)
     assert_parity(got, ref, dtype, "godunov")
     # Ghost layer untouched.
     for sl in (np.s_[:, 0], np.s_[:, -1], np.s_[:, :, 0], np.s_[:, :, -1], np.s_[..., 0], np.s_[..., -1]):
@@ -182,6 +182,8 @@ def test_error_paths(eb, ex):
         eb.set_godunov_config(0, 4, -1)
     with pytest.raises(EulerError):
         eb.set_godunov_config(3, 16, 8)
+    with pytest.raises(EulerError):
+        eb.set_godunov_config(4, 4, 8)  # paired rows: 8 or 16 rows per tile
     # linear tiles need rows of at most 64*by cells
     wide = random_prim((300, 4, 4), seed=26)
     eb.set_godunov_config(3, 4, 8)
@@ -190,8 +192,8 @@ def test_error_paths(eb, ex):
             eb.godunov(ex, to_prim(eb, wide), to_cons(eb, wide), eb.PerfectGas(1.4), eb.UniformMesh3d(1, 1, 1),
                        eb.hllc(), 0.1)
     finally:
-        eb.set_godunov_config(1, 0, 0)
-    assert eb.get_godunov_config() == (1, 0, 0)
+        eb.set_godunov_config(4, 0, 0)
+    assert eb.get_godunov_config() == (4, 0, 0)
     with pytest.raises(TypeError):
         eb.godunov(ex, P, to_cons(eb, prim.astype(np.float32)), eb.PerfectGas(1.4),
                    eb.UniformMesh3d(1, 1, 1), eb.hllc(), 0.1)
