@@ -5,9 +5,10 @@
 // godunov_tile in kernels.hpp), so every instance's results are bitwise those of a
 // launch of its own.
 //
-// Tiles are either 64 x BY columns of cells (layout 0) or runs of 64*BY
-// consecutive cells of a plane (layout 1, godunov_tile_lin), which lose no lanes
-// to ragged x extents.
+// Tiles are 64 x BY columns of cells (layout 0), runs of 64*BY consecutive cells
+// of a plane (layout 1, godunov_tile_lin), which lose no lanes to ragged x
+// extents, or 64 x BY columns with two rows per lane (layout 2,
+// godunov_tile_pair).
 //
 // Scheduling: the tile list is split into one contiguous range per XCD (ranges of
 // about equal cell count, instances kept together so halo re-reads stay in that
@@ -84,7 +85,8 @@ __device__ __forceinline__ int wave_uniform(int v)
 // dt is read from tune_dt (zeros), the conservatives are stored back unchanged,
 // the next dt lands in tune_out and no simulated time is accumulated.
 template <class T, class I, int BY, int Layout>
-__global__ __launch_bounds__(64 * BY) void k_batch_step(
+__global__ __launch_bounds__(Layout == 2 ? 32 * BY : 64 * BY)
+__attribute__((amdgpu_waves_per_eu(Layout == 2 ? 2 : 1))) void k_batch_step(
         BatchInstance<T> const* insts,
         BatchTile const* tiles,
         int const* ranges,
@@ -128,7 +130,16 @@ __global__ __launch_bounds__(64 * BY) void k_batch_step(
         T const dt = in.cfl * dt_in[0];
         T const dtodv = dt / in.dv;
         T lm;
-        if constexpr (Layout == 1) {
+        if constexpr (Layout == 2) {
+            // paired rows: 64 x BY tiles, BY/2 waves of two rows each
+            int const bx = tl.local % in.tiles_x;
+            int const by = (tl.local / in.tiles_x) % in.tiles_y;
+            int const bz = tl.local / (in.tiles_x * in.tiles_y);
+            lm = godunov_tile_pair<T, I, BY, true>(pin, in.cons, in.buf[1 - parity], static_cast<I>(in.n0),
+                                                   static_cast<I>(in.n1), static_cast<I>(in.n2), in.kchunk, bx, by,
+                                                   bz, in.gamma, in.gm1, in.ds0, in.ds1, in.ds2, dtodv, in.invdx0,
+                                                   in.invdx1, in.invdx2, !tune);
+        } else if constexpr (Layout == 1) {
             // linear tiles: tiles_x tiles per plane, tiles_y = 1
             lm = godunov_tile_lin<T, I, BY, true>(pin, in.cons, in.buf[1 - parity], static_cast<I>(in.n0),
                                                   static_cast<I>(in.n1), static_cast<I>(in.n2), in.kchunk,

@@ -16,7 +16,7 @@ struct eb_batch
     int real_bits = 64; // 64: double, 32: float
     int index_bits = 64;
     int ninst = 0;
-    int layout = 0; // 0: 64 x by tiles, 1: linear runs of 64*by cells
+    int layout = 0; // 0: 64 x by tiles, 1: linear runs of 64*by cells, 2: 64 x by tiles, two rows per lane
     int by = 8;
     int kchunk = 32;
     int ntiles = 0;
@@ -48,7 +48,9 @@ constexpr int kBatchQueues = 8; // one per XCD
 
 bool batch_by_supported(int layout, int by)
 {
-    return layout == 0 ? (by == 4 || by == 8 || by == 16) : (layout == 1 && (by == 4 || by == 8));
+    return layout == 0   ? (by == 4 || by == 8 || by == 16)
+           : layout == 1 ? (by == 4 || by == 8)
+                         : (layout == 2 && (by == 8 || by == 16));
 }
 
 template <class T>
@@ -76,7 +78,8 @@ int batch_resident_blocks(int* out)
     int per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess
         || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
-        || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eb::k_batch_step<T, I, BY, L>, 64 * BY, 0)
+        || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eb::k_batch_step<T, I, BY, L>,
+                                                        L == 2 ? 32 * BY : 64 * BY, 0)
                    != hipSuccess) {
         return fail(EB_ERR_HIP, "occupancy query failed");
     }
@@ -89,6 +92,9 @@ int batch_resident(int layout, int by, int* out)
 {
     if (layout == 1) {
         return by == 4 ? batch_resident_blocks<T, I, 4, 1>(out) : batch_resident_blocks<T, I, 8, 1>(out);
+    }
+    if (layout == 2) {
+        return by == 8 ? batch_resident_blocks<T, I, 8, 2>(out) : batch_resident_blocks<T, I, 16, 2>(out);
     }
     if (by == 4) {
         return batch_resident_blocks<T, I, 4, 0>(out);
@@ -353,7 +359,16 @@ int batch_launch(eb_batch* b, hipStream_t s, int parity, bool tune)
 #define EB_BATCH_LAUNCH(BY, L)                                                                                    \
     hipLaunchKernelGGL((eb::k_batch_step<T, I, BY, L>), grid, dim3(64, BY), 0, s, insts, b->tiles, b->ranges, \
                        b->nq, b->queues, parity, tdt, tout)
-    if (b->layout == 1) {
+#define EB_BATCH_LAUNCH_PAIR(BY)                                                                                  \
+    hipLaunchKernelGGL((eb::k_batch_step<T, I, BY, 2>), grid, dim3(64, BY / 2), 0, s, insts, b->tiles, b->ranges, \
+                       b->nq, b->queues, parity, tdt, tout)
+    if (b->layout == 2) {
+        if (b->by == 8) {
+            EB_BATCH_LAUNCH_PAIR(8);
+        } else {
+            EB_BATCH_LAUNCH_PAIR(16);
+        }
+    } else if (b->layout == 1) {
         if (b->by == 4) {
             EB_BATCH_LAUNCH(4, 1);
         } else {
@@ -367,6 +382,7 @@ int batch_launch(eb_batch* b, hipStream_t s, int parity, bool tune)
         EB_BATCH_LAUNCH(16, 0);
     }
 #undef EB_BATCH_LAUNCH
+#undef EB_BATCH_LAUNCH_PAIR
     return check_launch("batch_step");
 }
 
@@ -398,7 +414,7 @@ int batch_per_instance(eb_batch* b, hipStream_t s, int parity)
 template <class T, class I>
 int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best_ms)
 {
-    static constexpr int kShapes[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 4}, {1, 8}}; // (layout, by)
+    static constexpr int kShapes[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 4}, {1, 8}, {2, 8}, {2, 16}}; // (layout, by)
     static constexpr int kKc[] = {0, 64, 32, 16, 8};
     int64_t max_n0 = 0;
     for (int i = 0; i < b->ninst; ++i) {
@@ -528,7 +544,8 @@ int eb_batch_set_tiles(eb_batch* batch, int layout, int by, int kchunk)
         return fail(EB_ERR_INVALID_ARGUMENT, "null batch or negative kchunk");
     }
     if (!batch_by_supported(layout, by)) {
-        return fail(EB_ERR_UNSUPPORTED, "tiles: layout 0 with by 4, 8 or 16; layout 1 with by 4 or 8");
+        return fail(EB_ERR_UNSUPPORTED,
+                    "tiles: layout 0 with by 4, 8 or 16; layout 1 with by 4 or 8; layout 2 with by 8 or 16");
     }
     int const keep_layout = batch->layout;
     int const keep_by = batch->by;

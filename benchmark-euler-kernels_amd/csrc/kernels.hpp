@@ -736,6 +736,7 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_box(
 // rows and half the tile-edge face work per cell, two independent cells per lane,
 // and two barriers per plane for 64*BY cells. Same per-cell sums as the reference
 // (x, y, z order, godunov.hpp:48-82), so bitwise equal to the other variants.
+// `apply` = false leaves the conservatives' values unchanged (launch-shape timing).
 template <class T, class I, int BY, bool Fused>
 __device__ __forceinline__ T godunov_tile_pair(
         PrimPtrs<T const> const& prim,
@@ -756,7 +757,8 @@ __device__ __forceinline__ T godunov_tile_pair(
         T dtodv,
         T invdx0,
         T invdx1,
-        T invdx2)
+        T invdx2,
+        bool apply = true)
 {
     static_assert(BY % 2 == 0, "two rows per wave");
     static_assert(2 * 64 + 2 * BY <= 32 * BY, "one halo cell per thread at most (BY >= 8)");
@@ -949,8 +951,10 @@ __device__ __forceinline__ T godunov_tile_pair(
         accumulate(accB, ds0, fxB_hi, fxB_lo);
         accumulate(accB, ds1, fyB_hi, fAB);
         accumulate(accB, ds2, fzB_hi, fzB_lo);
-        apply_update(uA, dtodv, accA);
-        apply_update(uB, dtodv, accB);
+        if (apply) {
+            apply_update(uA, dtodv, accA);
+            apply_update(uB, dtodv, accB);
+        }
         if (validA) {
             st_cons(cons, plane, offA, uA);
             if constexpr (Fused) {

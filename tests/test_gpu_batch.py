@@ -80,7 +80,8 @@ def test_batch_matches_oracle_loop(eb, ex, orc, dtype, index_bits):
     ens.close()
 
 
-@pytest.mark.parametrize("tiles", [(4, 0, 0), (16, 8, 0), (8, 1, 0), (4, 3, 0), (4, 0, 1), (8, 5, 1), (4, 1, 1)])
+@pytest.mark.parametrize("tiles", [(4, 0, 0), (16, 8, 0), (8, 1, 0), (4, 3, 0), (4, 0, 1), (8, 5, 1), (4, 1, 1),
+                                   (8, 0, 2), (16, 3, 2), (8, 1, 2)])
 def test_batch_tile_layouts_are_bitwise_equal(eb, ex, orc, tiles):
     nsteps = 5
     print(f"batch tiles {tiles}", flush=True)
@@ -137,6 +138,10 @@ def test_batch_error_paths(eb, ex, orc):
         ens.set_tiles(8, -1)
     with pytest.raises(EulerError):
         ens.set_tiles(16, 8, layout=1)
+    with pytest.raises(EulerError):
+        ens.set_tiles(4, 8, layout=2)  # paired rows: 8 or 16 rows per tile
+    with pytest.raises(EulerError):
+        ens.set_tiles(8, 8, layout=3)
     ens.close()
     # linear tiles need every row to fit a tile: a 300-wide instance refuses them
     wide = [dict(inst[0])]
