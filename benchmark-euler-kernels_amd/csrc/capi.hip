@@ -273,17 +273,17 @@ void launch_tiled(
     }
 }
 
-// The fused step of the 64 x 8 tiled kernel on the box of interior cells [lo, hi]
-// (inclusive, per axis): the tiles that meet the box are marched over the box's
-// planes and only the box's cells are written. Used where only part of the grid
-// has to advance (the adjoint's recompute of a localised adjoint).
+// The fused step of the paired-row kernel (64 x 16 tiles) on the box of interior
+// cells [lo, hi] (inclusive, per axis): the tiles that meet the box are marched
+// over the box's planes and only the box's cells are written. Used where only part
+// of the grid has to advance (the adjoint's recompute of a localised adjoint).
 template <class T, class I>
 int launch_fused_region(hipStream_t s, eb::PrimPtrs<T const> const& pc, eb::ConsPtrs<T> const& cu,
                         eb::PrimPtrs<T> const& pout, int64_t n0, int64_t n1, int64_t n2, int64_t const* lo,
                         int64_t const* hi, T gamma, T ds0, T ds1, T ds2, eb::DtArg<T> const& dta, T invdx0, T invdx1,
                         T invdx2, eb::DtSink<T> const& sink)
 {
-    constexpr int BY = 8;
+    constexpr int BY = 16;
     int64_t const x0 = (lo[0] - 1) / 64;
     int64_t const x1 = (hi[0] - 1) / 64 + 1;
     int64_t const y0 = (lo[1] - 1) / BY;
@@ -300,7 +300,8 @@ int launch_fused_region(hipStream_t s, eb::PrimPtrs<T const> const& pc, eb::Cons
     }
     eb::CellBox<I> const box {static_cast<I>(lo[0]), static_cast<I>(hi[0]), static_cast<I>(lo[1]),
                               static_cast<I>(hi[1]), static_cast<I>(lo[2]), static_cast<I>(hi[2])};
-    hipLaunchKernelGGL((eb::k_godunov_box<T, I, BY>), dim3(static_cast<unsigned int>(nblk)), dim3(64, BY), 0, s, pc,
+    hipLaunchKernelGGL((eb::k_godunov_box_pair<T, I, BY>), dim3(static_cast<unsigned int>(nblk)), dim3(64, BY / 2), 0,
+                       s, pc,
                        cu, pout, static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk,
                        static_cast<int>(x1 - x0), static_cast<int>(y1 - y0), static_cast<int>(z1 - z0),
                        static_cast<int>(x0), static_cast<int>(y0), static_cast<int>(z0), box, gamma,

@@ -408,7 +408,7 @@ __device__ __forceinline__ void lds_barrier()
 // returns the thread's contribution to the next dt (largest finite value unless
 // Fused). `apply` = false leaves the conservatives' values unchanged while doing
 // the same loads, solves and stores (used to time launch shapes on live state).
-// Inclusive box of interior cells for the boxed form of the tile body.
+// Inclusive box of interior cells for the boxed form of the paired-row tile body.
 template <class I>
 struct CellBox
 {
@@ -420,7 +420,7 @@ struct CellBox
     I hi2;
 };
 
-template <class T, class I, int BY, bool Fused, bool Boxed = false>
+template <class T, class I, int BY, bool Fused>
 __device__ __forceinline__ T godunov_tile(
         PrimPtrs<T const> const& prim,
         ConsPtrs<T> const& cons,
@@ -441,8 +441,7 @@ __device__ __forceinline__ T godunov_tile(
         T invdx0,
         T invdx1,
         T invdx2,
-        bool apply,
-        CellBox<I> const& box = CellBox<I> {})
+        bool apply)
 {
     constexpr int BX = 64;
     constexpr int LX = BX + 2;
@@ -459,23 +458,14 @@ __device__ __forceinline__ T godunov_tile(
 
     I const i0 = 1 + static_cast<I>(bx) * BX;
     I const j0 = 1 + static_cast<I>(by) * BY;
-    I kbeg = 1 + static_cast<I>(bz) * kchunk;
-    I kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
-    if constexpr (Boxed) {
-        // the tile's planes inside the box (block-uniform; never empty for a tile
-        // of the box's tile range)
-        kbeg = kbeg > box.lo2 ? kbeg : box.lo2;
-        kend = kend < box.hi2 + 1 ? kend : box.hi2 + 1;
-    }
+    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
     I const s2 = n0 * n1;
 
     // Own cell, clamped into the array for loads (only valid cells are written).
     I const i = i0 + tx;
     I const j = j0 + ty;
-    bool valid = (i <= n0 - 2) && (j <= n1 - 2);
-    if constexpr (Boxed) {
-        valid = valid && i >= box.lo0 && i <= box.hi0 && j >= box.lo1 && j <= box.hi1;
-    }
+    bool const valid = (i <= n0 - 2) && (j <= n1 - 2);
     I const ic = (i < n0 - 1) ? i : n0 - 1;
     I const jc = (j < n1 - 1) ? j : n1 - 1;
     uint32_t const own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
@@ -680,51 +670,6 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     }
 }
 
-// The fused step restricted to a box of interior cells: tiles_x x tiles_y x
-// tiles_z tiles starting at tile (tile0_x, tile0_y, tile0_z) are marched, each
-// over the box's k range only, and only cells inside the box are written (the
-// others keep their values). The next dt is the minimum over the box.
-template <class T, class I, int BY>
-__global__ __launch_bounds__(64 * BY) void k_godunov_box(
-        PrimPtrs<T const> prim,
-        ConsPtrs<T> cons,
-        PrimPtrs<T> prim_out,
-        I n0,
-        I n1,
-        I n2,
-        int kchunk,
-        int tiles_x,
-        int tiles_y,
-        int tiles_z,
-        int tile0_x,
-        int tile0_y,
-        int tile0_z,
-        CellBox<I> box,
-        T gamma,
-        T gm1,
-        T ds0,
-        T ds1,
-        T ds2,
-        DtArg<T> dta,
-        T invdx0,
-        T invdx1,
-        T invdx2,
-        DtSink<T> sink)
-{
-    int const nblk = tiles_x * tiles_y * tiles_z;
-    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
-    int const bx = tile0_x + t % tiles_x;
-    int const by = tile0_y + (t / tiles_x) % tiles_y;
-    int const bz = tile0_z + t / (tiles_x * tiles_y);
-    T const local_min = godunov_tile<T, I, BY, true, true>(prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz,
-                                                            gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta), invdx0,
-                                                            invdx1, invdx2, true, box);
-    T const bm = block_min<T, BY>(local_min);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
-    }
-}
-
 // Paired-row Godunov update (2.5-D march along k), configuration variant 4.
 //
 // A block of BY/2 waves owns a 64 x BY column of interior cells; each lane owns two
@@ -737,7 +682,7 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_box(
 // and two barriers per plane for 64*BY cells. Same per-cell sums as the reference
 // (x, y, z order, godunov.hpp:48-82), so bitwise equal to the other variants.
 // `apply` = false leaves the conservatives' values unchanged (launch-shape timing).
-template <class T, class I, int BY, bool Fused>
+template <class T, class I, int BY, bool Fused, bool Boxed = false>
 __device__ __forceinline__ T godunov_tile_pair(
         PrimPtrs<T const> const& prim,
         ConsPtrs<T> const& cons,
@@ -758,7 +703,8 @@ __device__ __forceinline__ T godunov_tile_pair(
         T invdx0,
         T invdx1,
         T invdx2,
-        bool apply = true)
+        bool apply = true,
+        CellBox<I> const& box = CellBox<I> {})
 {
     static_assert(BY % 2 == 0, "two rows per wave");
     static_assert(2 * 64 + 2 * BY <= 32 * BY, "one halo cell per thread at most (BY >= 8)");
@@ -780,15 +726,26 @@ __device__ __forceinline__ T godunov_tile_pair(
 
     I const i0 = 1 + static_cast<I>(bx) * BX;
     I const j0 = 1 + static_cast<I>(by) * BY;
-    I const kbeg = 1 + static_cast<I>(bz) * kchunk;
-    I const kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    I kbeg = 1 + static_cast<I>(bz) * kchunk;
+    I kend = (kbeg + kchunk < n2 - 1) ? kbeg + kchunk : n2 - 1;
+    if constexpr (Boxed) {
+        // the tile's planes inside the box (block-uniform; never empty for a tile
+        // of the box's tile range)
+        kbeg = kbeg > box.lo2 ? kbeg : box.lo2;
+        kend = kend < box.hi2 + 1 ? kend : box.hi2 + 1;
+    }
     I const s2 = n0 * n1;
 
     I const i = i0 + tx;
     I const jA = j0 + 2 * ty;
     I const jB = jA + 1;
-    bool const validA = (i <= n0 - 2) && (jA <= n1 - 2);
-    bool const validB = (i <= n0 - 2) && (jB <= n1 - 2);
+    bool validA = (i <= n0 - 2) && (jA <= n1 - 2);
+    bool validB = (i <= n0 - 2) && (jB <= n1 - 2);
+    if constexpr (Boxed) {
+        bool const in_i = i >= box.lo0 && i <= box.hi0;
+        validA = validA && in_i && jA >= box.lo1 && jA <= box.hi1;
+        validB = validB && in_i && jB >= box.lo1 && jB <= box.hi1;
+    }
     I const ic = (i < n0 - 1) ? i : n0 - 1;
     I const jAc = (jA < n1 - 1) ? jA : n1 - 1;
     I const jBc = (jB < n1 - 1) ? jB : n1 - 1;
@@ -1024,6 +981,51 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
         if (threadIdx.x == 0 && threadIdx.y == 0) {
             grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
         }
+    }
+}
+
+// The fused step of the paired-row tiles restricted to a box of interior cells:
+// tiles_x x tiles_y x tiles_z tiles starting at tile (tile0_x, tile0_y, tile0_z)
+// are marched over the box's planes, and only cells inside the box are written
+// (the others keep their values). The next dt is the minimum over the box.
+template <class T, class I, int BY>
+__global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) void k_godunov_box_pair(
+        PrimPtrs<T const> prim,
+        ConsPtrs<T> cons,
+        PrimPtrs<T> prim_out,
+        I n0,
+        I n1,
+        I n2,
+        int kchunk,
+        int tiles_x,
+        int tiles_y,
+        int tiles_z,
+        int tile0_x,
+        int tile0_y,
+        int tile0_z,
+        CellBox<I> box,
+        T gamma,
+        T gm1,
+        T ds0,
+        T ds1,
+        T ds2,
+        DtArg<T> dta,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    int const nblk = tiles_x * tiles_y * tiles_z;
+    int const t = xcd_tile(static_cast<int>(blockIdx.x), nblk);
+    int const bx = tile0_x + t % tiles_x;
+    int const by = tile0_y + (t / tiles_x) % tiles_y;
+    int const bz = tile0_z + t / (tiles_x * tiles_y);
+    T const local_min = godunov_tile_pair<T, I, BY, true, true>(prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz,
+                                                                 gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta),
+                                                                 invdx0, invdx1, invdx2, true, box);
+    T const bm = block_min<T, BY / 2>(local_min);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
     }
 }
 
