@@ -91,7 +91,8 @@ def test_time_step_uniform_closed_form(eb, ex):
 @pytest.mark.parametrize("index_bits", [32, 64])
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("tile", [(0, 4, 64), (0, 2, 3), (1, 8, 64), (1, 4, 3), (1, 16, 1), (2, 4, 64), (2, 8, 3),
-                                  (2, 4, 1), (3, 4, 64), (3, 8, 3), (3, 4, 1), (1, 0, 0), (3, 0, 5), (4, 8, 64), (4, 8, 3), (4, 16, 1), (4, 16, 5), (1, 4, 64), (4, 0, 0)])
+                                  (2, 4, 1), (3, 4, 64), (3, 8, 3), (3, 4, 1), (1, 0, 0), (3, 0, 5), (4, 8, 64),
+                                  (4, 8, 3), (4, 16, 1), (4, 16, 5), (1, 4, 64), (4, 0, 0)])
 def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
     print(f"godunov dtype={dtype.__name__} bits={index_bits} shape={shape} tile={tile}", flush=True)
     gamma = 1.4

@@ -64,7 +64,8 @@ def test_fused_loop_int64_and_tiles(eb, ex, orc):
     prim, cons, dx = _setup(eb, ex, orc, "random", (67, 21, 10), np.float64)
     p_ref, c_ref = prim.copy(), cons.copy()
     orc.simulate(p_ref, c_ref, gamma, *dx, cfl, 4)
-    for tile in ((0, 2, 3), (0, 4, 64), (1, 4, 3), (1, 16, 5), (2, 4, 3), (2, 8, 64), (3, 4, 3), (3, 8, 64), (0, 0, 7), (4, 8, 3), (4, 16, 64), (1, 0, 0)):
+    for tile in ((0, 2, 3), (0, 4, 64), (1, 4, 3), (1, 16, 5), (2, 4, 3), (2, 8, 64), (3, 4, 3), (3, 8, 64), (0, 0, 7),
+                 (4, 8, 3), (4, 16, 64), (1, 0, 0)):
         eb.set_godunov_config(*tile)
         try:
             P, C = to_prim(eb, prim, 64), to_cons(eb, cons, 64)
