@@ -47,16 +47,22 @@ __device__ __forceinline__ T flux_normal(Flux<T> const& f)
     }
 }
 
-// Reciprocal and square root for the derivative arithmetic, which need not round
-// like the forward step (only the branch decisions must match, and those depend
-// on signs and on comparisons of values computed identically on both sides):
-// the hardware estimate refined by two Newton steps, to about 1 ulp.
+// Reciprocal for the derivative arithmetic, which need not round like the forward
+// step (only the branch decisions must match, and those depend on signs and on
+// comparisons of values computed identically on both sides): the hardware
+// estimate refined by EB_VJP_NEWTON Newton steps. Each step squares the relative
+// error; one step leaves about 1e-14, two about 1 ulp.
+#ifndef EB_VJP_NEWTON
+#define EB_VJP_NEWTON 1
+#endif
 __device__ __forceinline__ double fast_rcp(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return fma(r, fma(-x, r, 1.0), r);
+    for (int it = 0; it < EB_VJP_NEWTON; ++it) {
+        r = fma(r, fma(-x, r, 1.0), r);
+    }
+    return r;
 #else
     return 1.0 / x;
 #endif
