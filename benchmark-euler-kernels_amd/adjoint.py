@@ -10,8 +10,10 @@ Backward: conservative states are checkpointed every `checkpoint_every` steps;
 each segment is recomputed from its checkpoint (with the recorded dt, so it is
 bitwise the forward) and then reversed step by step through the hand-written HIP
 vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). The last segment's
-primitive states are kept by the forward pass itself. Memory: one state per
-checkpoint plus one primitive state per step of a segment.
+primitive states are kept by the forward pass itself. The recompute of the next
+segment runs on a second stream while the current segment's adjoint steps run,
+into a second set of segment buffers. Memory: one state per checkpoint plus two
+primitive states per step of a segment.
 
 Locality: the adjoint of P_n is zero outside the support box of dL/dP_N grown by
 N - n cells (one cell per step, the stencil's reach). Each adjoint step only
@@ -118,8 +120,9 @@ class Rollout:
              *self._geom(), self._slot_ptr(1))
 
     def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
-               dt_out: int, region=None):
-        args = (self.ex.context, self.ex.handle, self._p(src), self._p(cons), self._p(dst), *self._geom(),
+               dt_out: int, region=None, ex: Optional[ops.ExecSpace] = None):
+        ex = self.ex if ex is None else ex
+        args = (ex.context, ex.handle, self._p(src), self._p(cons), self._p(dst), *self._geom(),
                 self._cfl_arg, dt_in, self._slot_ptr(1), dt_out, None)
         if region is None:
             call(f"eb_step_fused_{self.suf}", *args)
@@ -180,19 +183,22 @@ class Rollout:
         self._tail_ready = keep
         return _clone_prim(self.prim_final)
 
-    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool):
-        """Grows the segment buffers to `need`. valid=True fills new buffers with a
-        valid state (P_0): a box-restricted recompute leaves the cells outside its
-        box as they were, and the adjoint steps read them with a zero adjoint
-        weight, which needs finite values. The forward pass writes every interior
-        cell of the buffers it asks for, so it passes valid=False."""
-        while len(self._segment) < need:
+    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool, bufs=None):
+        """Grows a set of segment buffers (the forward's set by default) to `need`.
+        valid=True fills new buffers with a valid state (P_0): a box-restricted
+        recompute leaves the cells outside its box as they were, and the adjoint
+        steps read them with a zero adjoint weight, which needs finite values. The
+        forward pass writes every interior cell of the buffers it asks for, so it
+        passes valid=False."""
+        bufs = self._segment if bufs is None else bufs
+        while len(bufs) < need:
             b = _clone_prim(self.prim0) if valid else _empty_prim_like(cons)
             self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
-            self._segment.append(b)
+            bufs.append(b)
 
     # --------------------------------------------------------------- backward
-    def _recompute(self, start: int, stop: int, needed=None) -> List[ops.EulerPrimArrays]:
+    def _recompute(self, start: int, stop: int, needed=None, bufs=None,
+                   ex: Optional[ops.ExecSpace] = None) -> List[ops.EulerPrimArrays]:
         """P_n for n in [start, stop), recomputed from the checkpoint at `start`
         with the recorded time steps (bitwise the forward's states). The backward
         visits each segment once, so the checkpoint itself becomes the working state
@@ -201,18 +207,22 @@ class Rollout:
         needed(n) -> region (six int64, inclusive interior box) or None: P_n is
         only required on that box. The boxes must shrink by at least one cell per
         step (needed(n+1) grown by one cell lies inside needed(n)); P_n is then
-        exact on needed(n) and holds some earlier valid state elsewhere."""
+        exact on needed(n) and holds some earlier valid state elsewhere.
+
+        bufs: the buffer set to fill (k valid buffers; the forward's set by
+        default); ex: the execution space (stream) to run on."""
+        ex = self.ex if ex is None else ex
+        bufs = self._segment if bufs is None else bufs
         cons = self.checkpoints.pop(start)
-        self._ensure_segment(cons, stop - start, valid=True)
         if start == 0:
             prims = [self.prim0]
         else:
-            ops.cons_to_prim(self.ex, cons, self._segment[0], self.eos)
-            prims = [self._segment[0]]
+            ops.cons_to_prim(ex, cons, bufs[0], self.eos)
+            prims = [bufs[0]]
         for n in range(start, stop - 1):
-            nxt = self._segment[n - start + 1]
+            nxt = bufs[n - start + 1]
             self._fused(prims[-1], cons, nxt, self._dt_ptr(n), self._slot_ptr(0),
-                        None if needed is None else needed(n + 1))
+                        None if needed is None else needed(n + 1), ex)
             prims.append(nxt)
         return prims
 
@@ -260,15 +270,48 @@ class Rollout:
         def needed(n):
             # the adjoint step of P_n reads P_n on its output box grown by one cell
             return grown(self.nsteps - n + 1)
-        starts = sorted(self.checkpoints) + [self.tail]  # consumed segment by segment below
-        for start in reversed(starts):
+        # Segments last to first. The next segment's states are recomputed on a side
+        # stream into the other of two buffer sets while this segment's adjoint
+        # steps run on the main stream; events order each set's reuse.
+        segs = list(reversed(sorted(self.checkpoints) + [self.tail]))
+        side = _side_exec(self.ex) if len(segs) > 1 else None
+        sets = [self._segment, []]
+        if len(segs) > 1:
+            any_cons = self.checkpoints[segs[1]]
+            self._ensure_segment(any_cons, self.k, valid=True)
+            self._ensure_segment(any_cons, self.k, valid=True, bufs=sets[1])
+        main = self.ex.stream
+        if side is not None:
+            # the side stream starts after everything issued so far on the main
+            # stream (the forward's checkpoints, the buffers set up above)
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.stream.wait_event(ready)
+        released = [None, None]  # event after the last adjoint step that read set j
+        keep = []  # states used on the side stream stay referenced until the end
+
+        def recompute_on_side(i):
+            start = segs[i]
             stop = min(start + self.k, self.nsteps)
-            if start == self.tail:
+            if released[i % 2] is not None:
+                side.stream.wait_event(released[i % 2])
+            keep.append(self.checkpoints[start])
+            prims = self._recompute(start, stop, needed, sets[i % 2], side)
+            done = torch.cuda.Event()
+            done.record(side.stream)
+            return prims, done
+
+        pending = None
+        for i, start in enumerate(segs):
+            stop = min(start + self.k, self.nsteps)
+            if i == 0:
                 # stored by the forward pass
                 head = self.prim0 if start == 0 else self._segment[0]
                 prims = [head] + self._segment[1:stop - start]
             else:
-                prims = self._recompute(start, stop, needed)
+                prims, done = pending
+                main.wait_event(done)
+            pending = recompute_on_side(i + 1) if i + 1 < len(segs) else None
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
                 dt = self._dt_ptr(n)
@@ -280,7 +323,25 @@ class Rollout:
                     call(f"eb_godunov_vjp_{suf}", h, self._p(P), self._p(mu), self._p(mu_next), *geom, dt,
                          self._cfl_arg, region())
                     mu, mu_next = mu_next, mu
+            released[i % 2] = torch.cuda.Event()
+            released[i % 2].record(main)
+        del keep
         return gprim0
+
+
+_SIDE: dict = {}
+
+
+def _side_exec(ex: ops.ExecSpace) -> ops.ExecSpace:
+    """A second execution space (own stream at the default priority, own reduction
+    scratch) on ex's device, kept for the process: the backward recomputes on it
+    while the adjoint steps run on ex's stream."""
+    key = (ex.device.index, ex.handle)
+    side = _SIDE.get(key)
+    if side is None:
+        side = ops.ExecSpace(ex.device, torch.cuda.Stream(ex.device))
+        _SIDE[key] = side
+    return side
 
 
 class _RolloutFunction(torch.autograd.Function):
