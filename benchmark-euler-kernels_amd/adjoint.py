@@ -120,7 +120,7 @@ class Rollout:
              *self._geom(), self._slot_ptr(1))
 
     def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
-               dt_out: int, region=None, ex: Optional[ops.ExecSpace] = None):
+               dt_out: Optional[int], region=None, ex: Optional[ops.ExecSpace] = None):
         ex = self.ex if ex is None else ex
         args = (ex.context, ex.handle, self._p(src), self._p(cons), self._p(dst), *self._geom(),
                 self._cfl_arg, dt_in, self._slot_ptr(1), dt_out, None)
@@ -221,8 +221,11 @@ class Rollout:
             prims = [bufs[0]]
         for n in range(start, stop - 1):
             nxt = bufs[n - start + 1]
-            self._fused(prims[-1], cons, nxt, self._dt_ptr(n), self._slot_ptr(0),
-                        None if needed is None else needed(n + 1), ex)
+            if needed is None:
+                self._fused(prims[-1], cons, nxt, self._dt_ptr(n), self._slot_ptr(0), None, ex)
+            else:
+                # the recorded time steps are replayed: the box step computes none
+                self._fused(prims[-1], cons, nxt, self._dt_ptr(n), None, needed(n + 1), ex)
             prims.append(nxt)
         return prims
 

@@ -300,12 +300,14 @@ int launch_fused_region(hipStream_t s, eb::PrimPtrs<T const> const& pc, eb::Cons
     }
     eb::CellBox<I> const box {static_cast<I>(lo[0]), static_cast<I>(hi[0]), static_cast<I>(lo[1]),
                               static_cast<I>(hi[1]), static_cast<I>(lo[2]), static_cast<I>(hi[2])};
-    hipLaunchKernelGGL((eb::k_godunov_box_pair<T, I, BY>), dim3(static_cast<unsigned int>(nblk)), dim3(64, BY / 2), 0,
-                       s, pc,
-                       cu, pout, static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk,
-                       static_cast<int>(x1 - x0), static_cast<int>(y1 - y0), static_cast<int>(z1 - z0),
-                       static_cast<int>(x0), static_cast<int>(y0), static_cast<int>(z0), box, gamma,
-                       static_cast<T>(gamma - 1), ds0, ds1, ds2, dta, invdx0, invdx1, invdx2, sink);
+    // Without dt_out the step computes no time step (callers that replay recorded ones).
+    auto const kernel = sink.dt_out != nullptr ? eb::k_godunov_box_pair<T, I, BY, true>
+                                               : eb::k_godunov_box_pair<T, I, BY, false>;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned int>(nblk)), dim3(64, BY / 2), 0, s, pc, cu, pout,
+                       static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), kchunk, static_cast<int>(x1 - x0),
+                       static_cast<int>(y1 - y0), static_cast<int>(z1 - z0), static_cast<int>(x0),
+                       static_cast<int>(y0), static_cast<int>(z0), box, gamma, static_cast<T>(gamma - 1), ds0, ds1,
+                       ds2, dta, invdx0, invdx1, invdx2, sink);
     return check_launch("step_fused_region");
 }
 
@@ -768,8 +770,11 @@ int abi_step_fused(eb_context* ctx, hipStream_t s, PS const* prim_in, CS const* 
         return fail(EB_ERR_INVALID_ARGUMENT, "the fused step needs at least one interior cell");
     }
     if (ctx == nullptr || prim_null(prim_in) || cons_null(cons) || prim_null(prim_out) || dt_in == nullptr
-        || dt_out == nullptr) {
+        || (dt_out == nullptr && region == nullptr)) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (dt_out == nullptr && time_acc != nullptr) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "time_acc needs dt_out (the time is accumulated by the dt reduction)");
     }
     if (prim_in->d == prim_out->d || dt_in == dt_out) {
         return fail(EB_ERR_INVALID_ARGUMENT, "prim_out / dt_out must not alias the inputs");

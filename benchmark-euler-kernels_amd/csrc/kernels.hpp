@@ -682,7 +682,9 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
 // and two barriers per plane for 64*BY cells. Same per-cell sums as the reference
 // (x, y, z order, godunov.hpp:48-82), so bitwise equal to the other variants.
 // `apply` = false leaves the conservatives' values unchanged (launch-shape timing).
-template <class T, class I, int BY, bool Fused, bool Boxed = false>
+// WithDt = false (fused form) skips the per-cell time-step candidates: for callers
+// that replay recorded time steps; the returned minimum is then meaningless.
+template <class T, class I, int BY, bool Fused, bool Boxed = false, bool WithDt = true>
 __device__ __forceinline__ T godunov_tile_pair(
         PrimPtrs<T const> const& prim,
         ConsPtrs<T> const& cons,
@@ -917,8 +919,10 @@ __device__ __forceinline__ T godunov_tile_pair(
             if constexpr (Fused) {
                 Prim<T> const q1 = cons_to_prim_pt(uA, gm1);
                 st_prim(prim_out, plane, offA, q1);
-                T const c1 = sound_speed(gamma, q1.d, q1.p);
-                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                if constexpr (WithDt) {
+                    T const c1 = sound_speed(gamma, q1.d, q1.p);
+                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                }
             }
         }
         if (validB) {
@@ -926,8 +930,10 @@ __device__ __forceinline__ T godunov_tile_pair(
             if constexpr (Fused) {
                 Prim<T> const q1 = cons_to_prim_pt(uB, gm1);
                 st_prim(prim_out, plane, offB, q1);
-                T const c1 = sound_speed(gamma, q1.d, q1.p);
-                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                if constexpr (WithDt) {
+                    T const c1 = sound_speed(gamma, q1.d, q1.p);
+                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                }
             }
         }
 
@@ -987,8 +993,9 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
 // The fused step of the paired-row tiles restricted to a box of interior cells:
 // tiles_x x tiles_y x tiles_z tiles starting at tile (tile0_x, tile0_y, tile0_z)
 // are marched over the box's planes, and only cells inside the box are written
-// (the others keep their values). The next dt is the minimum over the box.
-template <class T, class I, int BY>
+// (the others keep their values). With WithDt the next dt is the minimum over the
+// box; without, no time step is computed and the sink is not touched.
+template <class T, class I, int BY, bool WithDt>
 __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) void k_godunov_box_pair(
         PrimPtrs<T const> prim,
         ConsPtrs<T> cons,
@@ -1020,12 +1027,14 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
     int const bx = tile0_x + t % tiles_x;
     int const by = tile0_y + (t / tiles_x) % tiles_y;
     int const bz = tile0_z + t / (tiles_x * tiles_y);
-    T const local_min = godunov_tile_pair<T, I, BY, true, true>(prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz,
-                                                                 gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta),
-                                                                 invdx0, invdx1, invdx2, true, box);
-    T const bm = block_min<T, BY / 2>(local_min);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+    T const local_min = godunov_tile_pair<T, I, BY, true, true, WithDt>(
+            prim, cons, prim_out, n0, n1, n2, kchunk, bx, by, bz, gamma, gm1, ds0, ds1, ds2, resolve_dtodv(dta),
+            invdx0, invdx1, invdx2, true, box);
+    if constexpr (WithDt) {
+        T const bm = block_min<T, BY / 2>(local_min);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+        }
     }
 }
 

@@ -154,6 +154,8 @@ int eb_step_fused_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* pr
 /* The same step on the box of interior cells region = {i_lo, i_hi, j_lo, j_hi,
  * k_lo, k_hi} (inclusive, clipped to the interior): only the box's cells are
  * written, the others keep their values, and dt_out is the minimum over the box.
+ * dt_out may be null (then time_acc must be null too): no time step is computed,
+ * for callers that replay recorded time steps.
  * Exact on the box when prim_in is exact on the box grown by one cell and cons on
  * the box. For callers that only need part of the grid advanced, such as the
  * adjoint's recompute of a localised adjoint. No reference counterpart. */

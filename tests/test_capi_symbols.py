@@ -88,7 +88,7 @@ def test_cpp_headers_compile_standalone(tmp_path):
 def test_argument_checks_without_gpu():
     """Host-side argument validation answers before any device work: bad
     configurations, a negative fill count, a null fill target, an unknown real
-    width for the tile-height query."""
+    width for the tile-height query, a fused step without dt_out."""
     lib = ctypes.CDLL(str(LIB))
     lib.eb_last_error.restype = ctypes.c_char_p
     EB_OK = 0
@@ -114,3 +114,18 @@ def test_argument_checks_without_gpu():
     assert lib.eb_fill_f64(None, None, 10, 0.0) != EB_OK
     assert b"null" in lib.eb_last_error()
     assert lib.eb_fill_f64(None, None, 0, 0.0) == EB_OK  # nothing to do
+    # dt_out may be null only for the box step, and then without time_acc (the
+    # checks come before the context or any array is touched on the device)
+    fake = (ctypes.c_double * 8)()
+    ptrs = (ctypes.c_void_p * 5)(*[ctypes.addressof(fake)] * 5)
+    dt_in = ctypes.c_double(1.0)
+    region = (ctypes.c_int64 * 6)(1, 2, 1, 2, 1, 2)
+    acc = ctypes.c_double(0.0)
+    common = [ctypes.byref(fake), None, ptrs, ptrs, (ctypes.c_void_p * 5)(*[ctypes.addressof(fake) + 8] * 5),
+              ctypes.c_int64(4), ctypes.c_int64(4), ctypes.c_int64(4), 64, ctypes.c_double(1.4),
+              ctypes.c_double(0.5), ctypes.c_double(0.5), ctypes.c_double(0.5), ctypes.c_double(0.5),
+              ctypes.byref(dt_in), None, None]
+    assert lib.eb_step_fused_f64(*common, None) != EB_OK
+    assert b"null" in lib.eb_last_error()
+    assert lib.eb_step_fused_region_f64(*common, ctypes.byref(acc), region) != EB_OK
+    assert b"time_acc" in lib.eb_last_error()

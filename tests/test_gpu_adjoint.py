@@ -309,7 +309,9 @@ def test_step_fused_region_is_exact_on_its_box(eb, ex):
     """The box-restricted fused step (eb_step_fused_region_f64) writes exactly the
     box's cells, bitwise equal to the full step's, and leaves every other cell of
     the output primitives and of the conservatives as it was; its dt is the
-    minimum over the box (not below the full step's)."""
+    minimum over the box (not below the full step's). Without dt_out the box step
+    computes no time step and writes the same cells, and the context's reduction
+    scratch stays usable (the next full step's dt is unchanged)."""
     from benchmark_euler_kernels_amd._lib import call
 
     ops = eb.operators
@@ -324,20 +326,24 @@ def test_step_fused_region_is_exact_on_its_box(eb, ex):
     box = (60, 131, 3, 12, 5, 19)  # i_lo, i_hi, j_lo, j_hi, k_lo, k_hi
     sentinel = [torch.full_like(t, 7.0) for t in prim.tensors()]
     results = []
-    for region in (None, (ctypes.c_int64 * 6)(*box)):
+    for region, with_dt in ((None, True), ((ctypes.c_int64 * 6)(*box), True), ((ctypes.c_int64 * 6)(*box), False),
+                            (None, True)):
         cons = eb.EulerConsArrays(*[t.clone() for t in cons0.tensors()])
         out = eb.EulerPrimArrays(*[t.clone() for t in sentinel])
         dt_out = torch.zeros(1, dtype=torch.float64, device="cuda")
         args = (ex.context, ex.handle, ctypes.byref(ops.prim_ptrs(prim)), ctypes.byref(ops.cons_ptrs(cons)),
                 ctypes.byref(ops.prim_ptrs(out)), *shape, 32, ctypes.c_double(G), *map(ctypes.c_double, dx),
-                ctypes.c_double(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr(), None)
+                ctypes.c_double(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr() if with_dt else None, None)
         if region is None:
             call("eb_step_fused_f64", *args)
         else:
             call("eb_step_fused_region_f64", *args, region)
         torch.cuda.synchronize()
         results.append((torch.stack(cons.tensors()), torch.stack(out.tensors()), dt_out.item()))
-    (c_full, p_full, dt_full), (c_box, p_box, dt_box) = results
+    (c_full, p_full, dt_full), (c_box, p_box, dt_box), (c_nodt, p_nodt, dt_nodt), (c_again, p_again, dt_again) = results
+    assert torch.equal(c_nodt, c_box) and torch.equal(p_nodt, p_box)
+    assert dt_nodt == 0.0  # not written
+    assert torch.equal(c_again, c_full) and torch.equal(p_again, p_full) and dt_again == dt_full
     inside = torch.zeros(shape[2], shape[1], shape[0], dtype=torch.bool, device="cuda")
     inside[box[4]:box[5] + 1, box[2]:box[3] + 1, box[0]:box[1] + 1] = True
     assert torch.equal(p_box[:, inside], p_full[:, inside])
