@@ -105,7 +105,7 @@ void loop_checks(std::string const& tag)
     eb::prim_to_cons(ex, eb::as_const(prim3), cons3, eos);
 
     T const cfl = T(0.49);
-    long const nt = 12;
+    long const nt = 40; // the graph-replayed run covers one 32-step graph, pairs and an odd step
     double t_ref = 0;
     for (long it = 0; it < nt; ++it) {
         T const dt = eb::time_step(ex, eb::as_const(prim), eos, mesh);

@@ -102,6 +102,29 @@ def test_graph_replays_match_eager(eb, ex, orc):
     assert graphed.graph is not None
 
 
+def test_long_graph_replays_match_eager(eb, ex, orc):
+    """Runs long enough for the multi-pair graph (2*CHUNK_PAIRS steps per replay,
+    then pairs, then an odd last step): every run, with eager work and
+    synchronisation in between, equals the eager loop at the same step."""
+    gamma, cfl = 1.4, 0.49
+    prim, cons, dx = _setup(eb, ex, orc, "implode", (26, 22, 18), np.float64)
+    eos, mesh = eb.PerfectGas(gamma), eb.UniformMesh3d(*dx)
+    Pe, Ce = to_prim(eb, prim), to_cons(eb, cons)
+    eager = eb.simulation.Simulation(ex, Pe, Ce, eos, mesh, cfl)
+    Pg, Cg = to_prim(eb, prim), to_cons(eb, cons)
+    graphed = eb.simulation.Simulation(ex, Pg, Cg, eos, mesh, cfl)
+    chunk = 2 * eb.simulation.Simulation.CHUNK_PAIRS
+    for rep, n in enumerate((chunk + 5, chunk, 2 * chunk + 3)):
+        graphed.run(n, use_graph=True)
+        eager.run(n)
+        torch.cuda.synchronize()
+        _ = (Ce.d * 2.0).sum().item()
+        assert np.array_equal(to_host(Cg), to_host(Ce)), f"run {rep}"
+        assert np.array_equal(to_host(graphed.current_prim()), to_host(eager.current_prim())), f"run {rep}"
+        assert graphed.time.item() == eager.time.item()
+    assert graphed.chunk_graph is not None and graphed.graph is not None
+
+
 def test_graph_run_after_odd_step_count(eb, ex, orc):
     """A graph-replayed run that starts after an odd number of steps (the captured
     pair starts from buffer 0, so one eager step realigns it) equals the eager loop."""
