@@ -161,6 +161,35 @@ def test_rollout_gradient_matches_torch_reference(eb, ex, k):
     assert ref.abs().sum() > 0
 
 
+def test_rollout_gradient_fp32(eb, ex):
+    """The FP32 rollout and its adjoint (FP32 state, adjoint kernels and box
+    recompute) against the FP64 tensor reference of the same scheme: the loss to
+    FP32 rounding, the gradient to 1e-4 of its largest entry (normwise), and the
+    gradient bitwise independent of the checkpoint interval."""
+    shape = (18, 14, 12)
+    dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
+    nsteps, cfl = 6, 0.5
+    P0 = _smooth_state(shape, 1)
+    mask = _octant_mask(shape, dx)
+    grads = []
+    for k in (2, 100):
+        ts = [torch.from_numpy(P0[f].copy()).to("cuda", torch.float32).requires_grad_(True) for f in range(5)]
+        out = eb.adjoint.rollout(eb.EulerPrimArrays(*ts, index_bits=32), eb.PerfectGas(G), eb.UniformMesh3d(*dx),
+                                 cfl, nsteps, k, exec_space=ex)
+        loss = (out[0] * out[0] * mask).sum()
+        loss.backward()
+        grads.append(torch.stack([t.grad for t in ts]))
+    assert torch.equal(grads[0], grads[1])
+    ref_in = [torch.from_numpy(P0[f].copy()).cuda().requires_grad_(True) for f in range(5)]
+    PN = tr.rollout(ref_in, G, dx, cfl, nsteps)
+    ref_loss = (PN[0] * PN[0] * mask).sum()
+    ref_loss.backward()
+    ref = torch.stack([t.grad for t in ref_in])
+    assert abs(loss.item() - ref_loss.item()) <= 1e-5 * abs(ref_loss.item())
+    err = (grads[0].double() - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-4, err
+
+
 def test_rollout_checkpoint_interval_is_bitwise_invariant(eb, ex):
     shape = (18, 14, 12)
     dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
@@ -305,9 +334,10 @@ def test_vjp_region_is_exact_for_localised_adjoint(eb, ex, first):
     assert outs[0].abs().sum() > 0
 
 
-def test_step_fused_region_is_exact_on_its_box(eb, ex):
-    """The box-restricted fused step (eb_step_fused_region_f64) writes exactly the
-    box's cells, bitwise equal to the full step's, and leaves every other cell of
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_step_fused_region_is_exact_on_its_box(eb, ex, dtype):
+    """The box-restricted fused step (eb_step_fused_region_*, FP64 and FP32) writes
+    exactly the box's cells, bitwise equal to the full step's, and leaves every other cell of
     the output primitives and of the conservatives as it was; its dt is the
     minimum over the box (not below the full step's). Without dt_out the box step
     computes no time step and writes the same cells, and the context's reduction
@@ -318,11 +348,13 @@ def test_step_fused_region_is_exact_on_its_box(eb, ex):
     shape = (150, 21, 23)  # three 64-wide tiles in i, ragged in every axis
     dx = (1 / 148, 1 / 19, 1 / 21)
     P0 = _smooth_state(shape, 4)
-    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)])
+    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).to("cuda", dtype) for f in range(5)])
+    suf = "f64" if dtype == torch.float64 else "f32"
+    real = ctypes.c_double if dtype == torch.float64 else ctypes.c_float
     cons0 = eb.EulerConsArrays(*[torch.empty_like(t) for t in prim.tensors()])
     ops.prim_to_cons(ex, prim, cons0, eb.PerfectGas(G))
-    dt_in = torch.tensor([1e-3], dtype=torch.float64, device="cuda")
-    big = torch.tensor([1e300], dtype=torch.float64, device="cuda")
+    dt_in = torch.tensor([1e-3], dtype=dtype, device="cuda")
+    big = torch.tensor([1e30], dtype=dtype, device="cuda")
     box = (60, 131, 3, 12, 5, 19)  # i_lo, i_hi, j_lo, j_hi, k_lo, k_hi
     sentinel = [torch.full_like(t, 7.0) for t in prim.tensors()]
     results = []
@@ -330,14 +362,14 @@ def test_step_fused_region_is_exact_on_its_box(eb, ex):
                             (None, True)):
         cons = eb.EulerConsArrays(*[t.clone() for t in cons0.tensors()])
         out = eb.EulerPrimArrays(*[t.clone() for t in sentinel])
-        dt_out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        dt_out = torch.zeros(1, dtype=dtype, device="cuda")
         args = (ex.context, ex.handle, ctypes.byref(ops.prim_ptrs(prim)), ctypes.byref(ops.cons_ptrs(cons)),
-                ctypes.byref(ops.prim_ptrs(out)), *shape, 32, ctypes.c_double(G), *map(ctypes.c_double, dx),
-                ctypes.c_double(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr() if with_dt else None, None)
+                ctypes.byref(ops.prim_ptrs(out)), *shape, 32, real(G), *map(real, dx),
+                real(0.5), dt_in.data_ptr(), big.data_ptr(), dt_out.data_ptr() if with_dt else None, None)
         if region is None:
-            call("eb_step_fused_f64", *args)
+            call(f"eb_step_fused_{suf}", *args)
         else:
-            call("eb_step_fused_region_f64", *args, region)
+            call(f"eb_step_fused_region_{suf}", *args, region)
         torch.cuda.synchronize()
         results.append((torch.stack(cons.tensors()), torch.stack(out.tensors()), dt_out.item()))
     (c_full, p_full, dt_full), (c_box, p_box, dt_box), (c_nodt, p_nodt, dt_nodt), (c_again, p_again, dt_again) = results
