@@ -12,7 +12,8 @@
 //
 // Scheduling: the tile list is split into one contiguous range per XCD (ranges of
 // about equal cell count, instances kept together so halo re-reads stay in that
-// XCD's L2); block b serves queue b % nq and pulls tiles with an atomic counter. The
+// XCD's L2), each range ordered largest tile first so that the tiles pulled last are
+// small; block b serves queue b % nq and pulls tiles with an atomic counter. The
 // last block of a queue to finish re-arms the counter, so consecutive launches (and
 // graph replays) need no memset. Each instance's next dt is completed by the last
 // of its tiles through the instance's own self-resetting min scratch.

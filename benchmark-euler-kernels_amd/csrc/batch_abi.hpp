@@ -181,6 +181,23 @@ int batch_layout(eb_batch* b, int layout, int by, int kchunk)
             ranges[q++] = t + 1;
         }
     }
+    // Largest tiles first within each queue (tiles pulled late are the small ones,
+    // which shortens the tail of a step); equal costs keep their instance order.
+    {
+        std::vector<int> order(ntiles);
+        for (int t = 0; t < ntiles; ++t) {
+            order[t] = t;
+        }
+        for (int qq = 0; qq < nq; ++qq) {
+            std::stable_sort(order.begin() + ranges[qq], order.begin() + ranges[qq + 1],
+                             [&](int x, int y) { return cost[x] > cost[y]; });
+        }
+        std::vector<eb::BatchTile> sorted(ntiles);
+        for (int t = 0; t < ntiles; ++t) {
+            sorted[t] = tiles[order[t]];
+        }
+        tiles.swap(sorted);
+    }
     if (ntiles > b->tiles_cap) {
         if (b->tiles != nullptr) {
             (void)hipFree(b->tiles);
