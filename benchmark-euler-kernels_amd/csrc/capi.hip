@@ -174,6 +174,13 @@ eb::DtSink<T> plain_sink(eb_context* ctx, T* dt_out)
     return eb::DtSink<T> {ctx->scratch, dt_out, nullptr, nullptr, nullptr, T(0)};
 }
 
+template <class T>
+bool aligned16(eb::PrimPtrs<T const> const& p)
+{
+    auto a = [](void const* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+    return a(p.d) && a(p.p) && a(p.ux0) && a(p.ux1) && a(p.ux2);
+}
+
 template <class T, class I, class PS>
 int run_time_step(
         eb_context* ctx,
@@ -195,8 +202,13 @@ int run_time_step(
     T const invdx0 = T(1) / dx0;
     T const invdx1 = T(1) / dx1;
     T const invdx2 = T(1) / dx2;
-    hipLaunchKernelGGL((eb::k_time_step<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc, static_cast<I>(n),
-                       gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+    if (aligned16(pc)) {
+        hipLaunchKernelGGL((eb::k_time_step_vec<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc,
+                           static_cast<I>(n), gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+    } else {
+        hipLaunchKernelGGL((eb::k_time_step<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc, static_cast<I>(n),
+                           gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+    }
     return check_launch("time_step");
 }
 
@@ -212,13 +224,6 @@ struct Mesh
         dv = dx0 * dx1 * dx2;
     }
 };
-
-template <class T>
-bool aligned16(eb::PrimPtrs<T const> const& p)
-{
-    auto a = [](void const* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
-    return a(p.d) && a(p.p) && a(p.ux0) && a(p.ux1) && a(p.ux2);
-}
 
 template <int Variant, class T, class I, int BY, bool Fused>
 void launch_tiled(

@@ -101,6 +101,72 @@ __global__ __launch_bounds__(256) void k_time_step(
     }
 }
 
+// The same reduction with 16-byte non-temporal loads (the fields are read once):
+// each lane reads V = 16/sizeof(T) consecutive cells of every field per load, four
+// such groups in flight. Needs the five field pointers 16-byte aligned; the cells
+// past the last whole group are taken one per thread. The minimum is exact, so the
+// result is that of k_time_step.
+template <class T, class I>
+__global__ __launch_bounds__(256) void k_time_step_vec(
+        PrimPtrs<T const> prim,
+        I n,
+        T gamma,
+        T invdx0,
+        T invdx1,
+        T invdx2,
+        DtSink<T> sink)
+{
+    constexpr int V = 16 / static_cast<int>(sizeof(T));
+    typedef T Vec __attribute__((ext_vector_type(V)));
+    I const ng = n / V; // whole groups
+    I const stride = static_cast<I>(gridDim.x) * 256;
+    I g = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x);
+    T local = Limits<T>::max();
+    auto group = [&](I gi) {
+        Vec const d = __builtin_nontemporal_load(reinterpret_cast<Vec const*>(prim.d) + gi);
+        Vec const p = __builtin_nontemporal_load(reinterpret_cast<Vec const*>(prim.p) + gi);
+        Vec const u0 = __builtin_nontemporal_load(reinterpret_cast<Vec const*>(prim.ux0) + gi);
+        Vec const u1 = __builtin_nontemporal_load(reinterpret_cast<Vec const*>(prim.ux1) + gi);
+        Vec const u2 = __builtin_nontemporal_load(reinterpret_cast<Vec const*>(prim.ux2) + gi);
+        Prim<T> q[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            q[v] = Prim<T> {d[v], p[v], u0[v], u1[v], u2[v]};
+        }
+        return [=](T& acc) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                T const c = sound_speed(gamma, q[v].d, q[v].p);
+                acc = min_join(acc, T(1) / inv_dt_cell(q[v], c, invdx0, invdx1, invdx2));
+            }
+        };
+    };
+    for (; g + 3 * stride < ng; g += 4 * stride) {
+        auto const a0 = group(g);
+        auto const a1 = group(g + stride);
+        auto const a2 = group(g + 2 * stride);
+        auto const a3 = group(g + 3 * stride);
+        a0(local);
+        a1(local);
+        a2(local);
+        a3(local);
+    }
+    for (; g < ng; g += stride) {
+        group(g)(local);
+    }
+    // the last n % V cells
+    I const rem = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x);
+    if (rem < n - ng * V) {
+        Prim<T> const q = load_prim(prim, ng * V + rem);
+        T const c = sound_speed(gamma, q.d, q.p);
+        local = min_join(local, T(1) / inv_dt_cell(q, c, invdx0, invdx1, invdx2));
+    }
+    T const bm = block_min<T, 4>(local);
+    if (threadIdx.x == 0) {
+        grid_min_commit(sink, bm, gridDim.x);
+    }
+}
+
 // ------------------------------------------------------------------- init ---
 
 // Every element of one field set to a value (the init_from_value role of

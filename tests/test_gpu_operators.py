@@ -66,6 +66,32 @@ def test_time_step(eb, ex, orc, dtype, index_bits, shape):
     assert float(out.item()) == ref and float(out2.item()) == ref
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_time_step_large_and_misaligned(eb, ex, orc, dtype):
+    """Enough cells for four 16-byte groups in flight per thread (6.5 M cells, an
+    odd count, so a partial group is left), and the same fields at an offset of one
+    element (not 16-byte aligned, so the one-cell-per-load kernel runs): both equal
+    the oracle's dt."""
+    gamma = 1.4
+    dx = (1.0 / 254, 1.0 / 158, 1.0 / 159)
+    shape = (256, 160, 161)
+    prim = random_prim(shape, seed=14, dtype=dtype)
+    ref = orc.time_step(prim, gamma, *dx)
+    P = to_prim(eb, prim, 64)
+    assert all(t.data_ptr() % 16 == 0 for t in P.tensors())
+    got = eb.time_step(ex, P, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx))
+    assert got == ref
+    shifted = []
+    for f in range(5):
+        base = torch.empty(prim[f].size + 1, dtype=P.d.dtype, device="cuda")
+        view = base[1:].view(prim[f].shape)
+        view.copy_(torch.from_numpy(np.ascontiguousarray(prim[f])))
+        shifted.append(view)
+    Q = eb.EulerPrimArrays(*shifted, index_bits=64)
+    assert all(t.data_ptr() % 16 != 0 for t in Q.tensors())
+    assert eb.time_step(ex, Q, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx)) == ref
+
+
 def test_time_step_nan_cells_skipped(eb, ex, orc):
     """Kokkos::Min keeps the incumbent when the candidate is NaN (time_step.hpp:37)."""
     prim = random_prim((9, 8, 7), seed=13)
