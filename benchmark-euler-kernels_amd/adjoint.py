@@ -185,14 +185,20 @@ class Rollout:
 
     def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool, bufs=None):
         """Grows a set of segment buffers (the forward's set by default) to `need`.
-        valid=True fills new buffers with a valid state (P_0): a box-restricted
-        recompute leaves the cells outside its box as they were, and the adjoint
-        steps read them with a zero adjoint weight, which needs finite values. The
-        forward pass writes every interior cell of the buffers it asks for, so it
-        passes valid=False."""
+        valid=True fills new buffers with a finite state (rho = p = 1, u = 0): a
+        box-restricted recompute leaves the cells outside its box as they were, and
+        the adjoint steps read them with a zero adjoint weight, which needs finite
+        values (fills write each field once, where a copy of P_0 would also read it).
+        The forward pass writes every interior cell of the buffers it asks for, so
+        it passes valid=False."""
         bufs = self._segment if bufs is None else bufs
         while len(bufs) < need:
-            b = _clone_prim(self.prim0) if valid else _empty_prim_like(cons)
+            b = _empty_prim_like(cons)
+            if valid:
+                b.d.fill_(1.0)
+                b.p.fill_(1.0)
+                for u in (b.ux0, b.ux1, b.ux2):
+                    u.zero_()
             self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
             bufs.append(b)
 
