@@ -285,13 +285,13 @@ void launch_tiled(
 // 45 us with 8-plane chunks and 32 blocks); long chunks save the plane a block
 // re-reads below its chunk. Measured lmax: 128 planes in FP64 (128^3: 8 planes;
 // 320^3: 64; 512^3: 128), 32 in FP32 (faster at 320^3, level at 256^3 and 512^3).
-int64_t pair_kchunk(int64_t columns, int64_t nz, int64_t lmax)
+int64_t pair_kchunk(int64_t columns, int64_t nz, int64_t lmax, int64_t slots = 256)
 {
     if (nz < 1) {
         return 1;
     }
     for (int64_t rounds = 1;; ++rounds) {
-        int64_t const per = std::max<int64_t>(1, rounds * 256 / columns); // chunks per column
+        int64_t const per = std::max<int64_t>(1, rounds * slots / columns); // chunks per column
         int64_t const kc = (nz + per - 1) / per;
         if (kc <= lmax || per >= nz) {
             return kc;
@@ -975,8 +975,9 @@ int abi_godunov_vjp(hipStream_t s, PS const* prim, CS const* mu_in, CS const* mu
         int64_t const y1 = (hi[1] - 1) / kBy + 1;
         int64_t const columns = (x1 - x0) * (y1 - y0);
         int64_t const nz = hi[2] - lo[2] + 1;
-        int64_t kc = nz * columns / 1024; // about 1024 blocks
-        kc = kc < 4 ? 4 : (kc > 64 ? 64 : kc);
+        // two blocks per CU (512 at a time), chunks of at most 8 planes (1024 blocks at
+        // 128^3); small boxes get one round of short chunks
+        int64_t const kc = pair_kchunk(columns, nz, 8, 512);
         kchunk = static_cast<int>(kc < n2 - 2 ? kc : n2 - 2);
         int64_t const z0 = (lo[2] - 1) / kchunk;
         int64_t const z1 = (hi[2] - 1) / kchunk + 1;
