@@ -203,8 +203,9 @@ int run_time_step(
     T const invdx1 = T(1) / dx1;
     T const invdx2 = T(1) / dx2;
     if (aligned16(pc)) {
-        hipLaunchKernelGGL((eb::k_time_step_vec<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc,
-                           static_cast<I>(n), gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+        // sized in groups of 16 bytes, so that each thread has four groups in flight
+        hipLaunchKernelGGL((eb::k_time_step_vec<T, I>), dim3(reduction_grid(n / (16 / int64_t(sizeof(T))))), dim3(256),
+                           0, s, pc, static_cast<I>(n), gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
     } else {
         hipLaunchKernelGGL((eb::k_time_step<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc, static_cast<I>(n),
                            gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
