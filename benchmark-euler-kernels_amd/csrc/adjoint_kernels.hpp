@@ -124,7 +124,7 @@ __device__ __forceinline__ void vjp_ghost_cells(
     I const cols = inner_k * inner_j * nkx;
     I const total = planes + rows + cols;
     auto inside = [](I a, I n) { return a >= 1 && a <= n - 2; };
-    for (I g = first; g < total; g += stride) {
+    for (I g = first; g < total; g = grid_next(g, total, stride)) {
         I i;
         I j;
         I k;
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_c2p_vjp(PrimPtrs<T const> prim, PrimPtr
                                                  I n, T gm1)
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I c = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); c < n; c += stride) {
+    for (I c = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); c < n; c = grid_next(c, n, stride)) {
         Prim<T> const g = load_prim(gprim, c);
         Cons<T> const r = c2p_vjp(load_prim(prim, c), PrimGrad<T> {g.d, g.p, g.u0, g.u1, g.u2}, gm1);
         store_cons(gcons, c, r);
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_p2c_vjp(PrimPtrs<T const> prim, ConsPtr
                                                  I n, T gm1)
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I c = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); c < n; c += stride) {
+    for (I c = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); c < n; c = grid_next(c, n, stride)) {
         PrimGrad<T> const r = p2c_vjp(load_prim(prim, c), load_cons(gcons, c), gm1);
         store_prim(gprim, c, Prim<T> {r.d, r.p, r.u0, r.u1, r.u2});
     }

@@ -78,9 +78,11 @@ int check_launch(char const* what)
 // Blocks of the FP64 diagnostic reductions (partials held by the context).
 constexpr int kDiagBlocks = 2048;
 
-// Largest total cell count accepted with 32-bit indices: the reference's int_cast
-// rule plus headroom for the grid-stride arithmetic (idx + 4*stride, stride <= 2^20).
-constexpr int64_t kMax32 = std::numeric_limits<int32_t>::max() - (int64_t(1) << 22);
+// Largest total cell count accepted with 32-bit indices. Grid-stride loops step
+// with grid_next, which never forms an index past n; the 2^22 cells below 2^31 - 1
+// are a margin for any other index arithmetic near the end of the arrays.
+constexpr int64_t kMax32 = EB_MAX_CELLS_INDEX32;
+static_assert(kMax32 <= std::numeric_limits<int32_t>::max() - (int64_t(1) << 22));
 
 int check_extents(int64_t n0, int64_t n1, int64_t n2, int index_bits)
 {

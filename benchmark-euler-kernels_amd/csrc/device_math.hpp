@@ -54,6 +54,15 @@ struct Flux
     T d, e, m0, m1, m2;
 };
 
+// Next index of a grid-stride loop, or n once the stride would reach past n.
+// idx + stride is formed only when it stays below n, so a 32-bit index cannot
+// overflow however large the grid is (n itself fits the index type).
+template <class I>
+__device__ __forceinline__ I grid_next(I idx, I n, I stride)
+{
+    return n - idx > stride ? idx + stride : n;
+}
+
 // Ternary min/max with the std:: semantics Kokkos::min/max follow.
 template <class T>
 __device__ __forceinline__ T tmin(T a, T b)

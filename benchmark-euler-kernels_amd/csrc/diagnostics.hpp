@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_boundary_partials(PrimPtrs<T c
     I const total = 2 * (fx + fy + fz);
     double const dtodv = static_cast<double>(resolve_dtodv(dta));
     I const stride = static_cast<I>(gridDim.x) * kDiagThreads;
-    for (I g = static_cast<I>(blockIdx.x) * kDiagThreads + threadIdx.x; g < total; g += stride) {
+    for (I g = static_cast<I>(blockIdx.x) * kDiagThreads + threadIdx.x; g < total; g = grid_next(g, total, stride)) {
         Flux<T> f;
         double w;
         if (g < 2 * fx) {

@@ -47,7 +47,7 @@ template <class T, class I>
 __global__ __launch_bounds__(256) void k_cons_to_prim(ConsPtrs<T const> cons, PrimPtrs<T> prim, I n, T gm1)
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
+    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx = grid_next(idx, n, stride)) {
         store_prim(prim, idx, cons_to_prim_pt(load_cons(cons, idx), gm1));
     }
 }
@@ -57,7 +57,7 @@ template <class T, class I>
 __global__ __launch_bounds__(256) void k_prim_to_cons(PrimPtrs<T const> prim, ConsPtrs<T> cons, I n, T gm1)
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
+    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx = grid_next(idx, n, stride)) {
         store_cons(cons, idx, prim_to_cons_pt(load_prim(prim, idx), gm1));
     }
 }
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_time_step(
     I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x);
     T local = Limits<T>::max();
     // Four cells in flight per thread to keep enough loads outstanding.
-    for (; idx + 3 * stride < n; idx += 4 * stride) {
+    for (; n - idx > 3 * stride; idx = grid_next(idx, n, 4 * stride)) {
         Prim<T> q[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_time_step(
             local = min_join(local, T(1) / inv_dt_cell(q[u], c, invdx0, invdx1, invdx2));
         }
     }
-    for (; idx < n; idx += stride) {
+    for (; idx < n; idx = grid_next(idx, n, stride)) {
         Prim<T> const q = load_prim(prim, idx);
         T const c = sound_speed(gamma, q.d, q.p);
         local = min_join(local, T(1) / inv_dt_cell(q, c, invdx0, invdx1, invdx2));
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_time_step_vec(
             }
         };
     };
-    for (; g + 3 * stride < ng; g += 4 * stride) {
+    for (; ng - g > 3 * stride; g = grid_next(g, ng, 4 * stride)) {
         auto const a0 = group(g);
         auto const a1 = group(g + stride);
         auto const a2 = group(g + 2 * stride);
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_time_step_vec(
         a2(local);
         a3(local);
     }
-    for (; g < ng; g += stride) {
+    for (; g < ng; g = grid_next(g, ng, stride)) {
         group(g)(local);
     }
     // the last n % V cells
@@ -175,7 +175,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_fill(T* a, int64_t n, T v)
 {
     int64_t const stride = static_cast<int64_t>(gridDim.x) * 256;
-    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; idx < n; idx += stride) {
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; idx < n; idx = grid_next(idx, n, stride)) {
         a[idx] = v;
     }
 }
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_init_implode(PrimPtrs<T> prim, I n0, I 
 {
     I const n = n0 * n1 * n2;
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
+    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx = grid_next(idx, n, stride)) {
         I const i = idx % n0;
         I const j = (idx / n0) % n1;
         I const k = idx / (n0 * n1);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_init_diagonal(
 {
     I const n = n0 * n1 * n2;
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
+    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx = grid_next(idx, n, stride)) {
         I const i = idx % n0;
         I const j = (idx / n0) % n1;
         I const k = idx / (n0 * n1);
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_init_random(
         double u_sigma)
 {
     I const stride = static_cast<I>(gridDim.x) * 256;
-    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx += stride) {
+    for (I idx = static_cast<I>(blockIdx.x) * 256 + static_cast<I>(threadIdx.x); idx < n; idx = grid_next(idx, n, stride)) {
         unsigned long long const g = static_cast<unsigned long long>(idx);
         double const d = d_lo + (d_hi - d_lo) * uniform01(seed, 0, g);
         double const p = p_lo + (p_hi - p_lo) * uniform01(seed, 1, g);
@@ -1833,7 +1833,7 @@ __global__ __launch_bounds__(256) void k_ghost_refresh(
     I const total = planes + rows + cols;
     I const stride = static_cast<I>(gridDim.x) * 256;
     T local = Limits<T>::max();
-    for (I g = static_cast<I>(blockIdx.x) * 256 + threadIdx.x; g < total; g += stride) {
+    for (I g = static_cast<I>(blockIdx.x) * 256 + threadIdx.x; g < total; g = grid_next(g, total, stride)) {
         I i;
         I j;
         I k;

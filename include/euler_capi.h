@@ -7,10 +7,13 @@
  * separate fields per state, layout_left, linear index i + n0*(j + n1*k), extents
  * n0 x n1 x n2 INCLUDING the one-cell ghost layer. The caller owns every array.
  *
- * index_bits selects the kernels' index type: 32 (the reference's `int` /
- * `unsigned` index_t, benchmarks/index_type.hpp:3-4) or 64 (int64_t). With 32 the
- * total cell count must fit a signed 32-bit index (the reference's int_cast rule,
- * benchmarks/benchmark_utils.hpp:10-17), else EB_ERR_INDEX_OVERFLOW.
+ * index_bits selects the kernels' index type: 32 (signed 32-bit) or 64 (int64_t).
+ * With 32 the total cell count must not exceed EB_MAX_CELLS_INDEX32, else
+ * EB_ERR_INDEX_OVERFLOW. The index width never changes a result, so a caller whose
+ * index type reaches further (the reference's `int` up to 2^31-1 cells, `unsigned`
+ * up to 2^32-1, benchmarks/index_type.hpp:3-4 and the int_cast rule of
+ * benchmarks/benchmark_utils.hpp:10-17) passes 64 above that limit; the C++ API
+ * in include/euler/ does so.
  *
  * Each function exists for f64 (real_t = double) and f32 (real_t = float), the
  * reference's real_type switch (benchmarks/real_type.hpp:3-4).
@@ -25,6 +28,10 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Largest total cell count the 32-bit index kernels accept: 2^31 - 1 less 2^22
+ * cells kept free so that no index formed inside a kernel can pass 2^31 - 1. */
+#define EB_MAX_CELLS_INDEX32 ((int64_t)2147483647 - ((int64_t)1 << 22))
 
 typedef enum eb_status {
     EB_OK = 0,
