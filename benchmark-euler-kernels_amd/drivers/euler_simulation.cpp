@@ -4,7 +4,7 @@
 // every output_freq steps, and the cell-update rate printed at the end.
 //
 //   euler_simulation [--nx 128] [--nt 200] [--output-freq 10] [--cfl 0.49]
-//                    [--gamma 1.4] [--real f64|f32] [--index int|long]
+//                    [--gamma 1.4] [--real f64|f32] [--index int|unsigned|long]
 //                    [--loop reference|fused|graph] [--init implode|diagonal]
 //                    [--out DIR]
 //
@@ -169,7 +169,13 @@ int main(int argc, char** argv)
         if (o.real == "f32" && o.index == "long") {
             return run<float, std::int64_t>(o);
         }
-        throw std::invalid_argument("--real f64|f32, --index int|long");
+        if (o.real == "f64" && o.index == "unsigned") {
+            return run<double, unsigned>(o);
+        }
+        if (o.real == "f32" && o.index == "unsigned") {
+            return run<float, unsigned>(o);
+        }
+        throw std::invalid_argument("--real f64|f32, --index int|unsigned|long");
     } catch (std::exception const& e) {
         std::cerr << "euler_simulation: " << e.what() << '\n';
         return 1;

@@ -19,7 +19,7 @@ void cons_to_prim(
     auto const p = detail::ptrs(prim_arrays);
     detail::check(detail::capi<T>::cons_to_prim(exec_space.stream(), &c, &p, detail::ext(prim_arrays, 0),
                                                 detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                                detail::index_bits<IndexType>(), eos.gamma()));
+                                                detail::index_bits<IndexType>(prim_arrays), eos.gamma()));
 }
 
 } // namespace eb

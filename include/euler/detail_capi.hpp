@@ -4,6 +4,8 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
+#include <limits>
 #include <type_traits>
 
 #include "../euler_capi.h"
@@ -40,13 +42,37 @@ EB_CAPI_TRAITS(double, f64)
 EB_CAPI_TRAITS(float, f32)
 #undef EB_CAPI_TRAITS
 
-// 64-bit index types use the int64 kernels; narrower ones the 32-bit kernels,
-// which reject extents whose cell count does not fit a signed 32-bit index.
+// Index width of the kernels for arrays of the given extents. 64-bit index types
+// use the int64 kernels. Narrower ones use the 32-bit kernels up to
+// EB_MAX_CELLS_INDEX32 cells, and the int64 kernels (same results) above it while
+// the cell count still fits IndexType: `int` reaches 2^31 - 1 cells and `unsigned`
+// 2^32 - 1, as with the reference's index_t (benchmarks/index_type.hpp:3-4). A
+// count past IndexType is sent to the 32-bit kernels, which reject it with
+// EB_ERR_INDEX_OVERFLOW.
 template <class IndexType>
-constexpr int index_bits() noexcept
+int index_bits(int64_t n0, int64_t n1, int64_t n2) noexcept
 {
     static_assert(std::is_integral_v<IndexType>, "IndexType must be an integer type");
-    return sizeof(IndexType) >= 8 ? 64 : 32;
+    if constexpr (sizeof(IndexType) >= 8) {
+        return 64;
+    } else {
+        if (n0 < 0 || n1 < 0 || n2 < 0) {
+            return 32; // rejected by the C ABI as an invalid argument
+        }
+        unsigned __int128 const total = static_cast<unsigned __int128>(n0) * static_cast<unsigned __int128>(n1)
+                                        * static_cast<unsigned __int128>(n2);
+        if (total <= static_cast<unsigned __int128>(EB_MAX_CELLS_INDEX32)) {
+            return 32;
+        }
+        return total <= static_cast<unsigned __int128>(std::numeric_limits<IndexType>::max()) ? 64 : 32;
+    }
+}
+
+template <class IndexType, class A>
+int index_bits(A const& arrays) noexcept
+{
+    return index_bits<IndexType>(static_cast<int64_t>(arrays.d.extent(0)), static_cast<int64_t>(arrays.d.extent(1)),
+                                 static_cast<int64_t>(arrays.d.extent(2)));
 }
 
 template <class E, class X>

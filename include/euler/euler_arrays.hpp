@@ -7,12 +7,14 @@
 #include <cstddef>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_runtime_api.h>
 
 #include "../euler_capi.h"
 #include "exec_space.hpp"
+#include "macros.hpp"
 #include "mdspan.hpp"
 
 namespace eb {
@@ -47,34 +49,40 @@ struct EulerFlux
     T mx2;
 };
 
-// Host-side point formulas in the reference's operation order (euler_arrays.hpp:38-75).
+// Point formulas in the reference's operation order (euler_arrays.hpp:38-75),
+// callable from host and device code.
 template <class T>
-T kinetic_energy(EulerPrim<T> const& prim) noexcept
+EB_FUNCTION T kinetic_energy(EulerPrim<T> const& prim) noexcept
 {
+    EB_IEEE_ORDER
     return prim.d * ((prim.ux0 * prim.ux0) + (prim.ux1 * prim.ux1) + (prim.ux2 * prim.ux2)) / 2;
 }
 
 template <class T>
-T kinetic_energy(EulerCons<T> const& cons) noexcept
+EB_FUNCTION T kinetic_energy(EulerCons<T> const& cons) noexcept
 {
+    EB_IEEE_ORDER
     return ((cons.mx0 * cons.mx0) + (cons.mx1 * cons.mx1) + (cons.mx2 * cons.mx2)) / cons.d / 2;
 }
 
 template <class T>
-T internal_energy(EulerCons<T> const& cons) noexcept
+EB_FUNCTION T internal_energy(EulerCons<T> const& cons) noexcept
 {
+    EB_IEEE_ORDER
     return cons.e - kinetic_energy(cons);
 }
 
 template <class T>
-EulerCons<T> to_cons(EulerPrim<T> const& prim, T const int_e) noexcept
+EB_FUNCTION EulerCons<T> to_cons(EulerPrim<T> const& prim, T const int_e) noexcept
 {
+    EB_IEEE_ORDER
     return {prim.d, kinetic_energy(prim) + int_e, prim.d * prim.ux0, prim.d * prim.ux1, prim.d * prim.ux2};
 }
 
 template <class T>
-EulerPrim<T> to_prim(EulerCons<T> const& cons, T const p) noexcept
+EB_FUNCTION EulerPrim<T> to_prim(EulerCons<T> const& cons, T const p) noexcept
 {
+    EB_IEEE_ORDER
     T const vol_spe = 1 / cons.d;
     return {cons.d, p, cons.mx0 * vol_spe, cons.mx1 * vol_spe, cons.mx2 * vol_spe};
 }
@@ -222,6 +230,95 @@ template <class E, class X, class L>
 EulerConsArrays<E*> data_handle(EulerConsArrays<mdspan<E, X, L>> const& a) noexcept
 {
     return {a.d.data_handle(), a.e.data_handle(), a.mx0.data_handle(), a.mx1.data_handle(), a.mx2.data_handle()};
+}
+
+// One cell of the five fields as a point struct, and back (euler_arrays.hpp:144-214,
+// 295-365): through rank-3 views at (i, j, k), or through raw field pointers at a
+// linear index. Callable from host code on host memory and from device code.
+template <class E, class I, std::size_t E0, std::size_t E1, std::size_t E2, class L>
+EB_FUNCTION EulerPrim<std::remove_const_t<E>> load(EulerPrimArrays<mdspan<E, extents<I, E0, E1, E2>, L>> const& a,
+                                                   I const i, I const j, I const k) noexcept
+{
+    return {a.d(i, j, k), a.p(i, j, k), a.ux0(i, j, k), a.ux1(i, j, k), a.ux2(i, j, k)};
+}
+
+template <class E, class I, std::size_t E0, std::size_t E1, std::size_t E2, class L>
+EB_FUNCTION EulerCons<std::remove_const_t<E>> load(EulerConsArrays<mdspan<E, extents<I, E0, E1, E2>, L>> const& a,
+                                                   I const i, I const j, I const k) noexcept
+{
+    return {a.d(i, j, k), a.e(i, j, k), a.mx0(i, j, k), a.mx1(i, j, k), a.mx2(i, j, k)};
+}
+
+template <class E, class I>
+EB_FUNCTION EulerPrim<std::remove_const_t<E>> load(EulerPrimArrays<E*> const& a, I const i) noexcept
+{
+    return {a.d[i], a.p[i], a.ux0[i], a.ux1[i], a.ux2[i]};
+}
+
+template <class E, class I>
+EB_FUNCTION EulerCons<std::remove_const_t<E>> load(EulerConsArrays<E*> const& a, I const i) noexcept
+{
+    return {a.d[i], a.e[i], a.mx0[i], a.mx1[i], a.mx2[i]};
+}
+
+template <class T, class I, std::size_t E0, std::size_t E1, std::size_t E2, class L>
+EB_FUNCTION void store(EulerPrim<T> const& q, EulerPrimArrays<mdspan<T, extents<I, E0, E1, E2>, L>> const& a,
+                       I const i, I const j, I const k) noexcept
+{
+    a.d(i, j, k) = q.d;
+    a.p(i, j, k) = q.p;
+    a.ux0(i, j, k) = q.ux0;
+    a.ux1(i, j, k) = q.ux1;
+    a.ux2(i, j, k) = q.ux2;
+}
+
+template <class T, class I, std::size_t E0, std::size_t E1, std::size_t E2, class L>
+EB_FUNCTION void store(EulerCons<T> const& u, EulerConsArrays<mdspan<T, extents<I, E0, E1, E2>, L>> const& a,
+                       I const i, I const j, I const k) noexcept
+{
+    a.d(i, j, k) = u.d;
+    a.e(i, j, k) = u.e;
+    a.mx0(i, j, k) = u.mx0;
+    a.mx1(i, j, k) = u.mx1;
+    a.mx2(i, j, k) = u.mx2;
+}
+
+template <class T, class I>
+EB_FUNCTION void store(EulerPrim<T> const& q, EulerPrimArrays<T*> const& a, I const i) noexcept
+{
+    a.d[i] = q.d;
+    a.p[i] = q.p;
+    a.ux0[i] = q.ux0;
+    a.ux1[i] = q.ux1;
+    a.ux2[i] = q.ux2;
+}
+
+template <class T, class I>
+EB_FUNCTION void store(EulerCons<T> const& u, EulerConsArrays<T*> const& a, I const i) noexcept
+{
+    a.d[i] = u.d;
+    a.e[i] = u.e;
+    a.mx0[i] = u.mx0;
+    a.mx1[i] = u.mx1;
+    a.mx2[i] = u.mx2;
+}
+
+// The same slices of the five fields (euler_arrays.hpp:106-117, 257-268): one
+// slice per rank, as eb::submdspan takes them.
+template <class E, class X, class L, class... S>
+auto subspan(EulerPrimArrays<mdspan<E, X, L>> const& a, S const&... slices)
+{
+    using Sub = decltype(submdspan(a.d, slices...));
+    return EulerPrimArrays<Sub> {submdspan(a.d, slices...), submdspan(a.p, slices...), submdspan(a.ux0, slices...),
+                                 submdspan(a.ux1, slices...), submdspan(a.ux2, slices...)};
+}
+
+template <class E, class X, class L, class... S>
+auto subspan(EulerConsArrays<mdspan<E, X, L>> const& a, S const&... slices)
+{
+    using Sub = decltype(submdspan(a.d, slices...));
+    return EulerConsArrays<Sub> {submdspan(a.d, slices...), submdspan(a.e, slices...), submdspan(a.mx0, slices...),
+                                 submdspan(a.mx1, slices...), submdspan(a.mx2, slices...)};
 }
 
 // Elements per field and bytes of the five fields (euler_arrays.hpp:367-399).

@@ -26,7 +26,7 @@ void godunov(
     auto const c = detail::ptrs(cons_arrays);
     detail::check(detail::capi<T>::godunov(exec_space.stream(), &p, &c, detail::ext(prim_arrays, 0),
                                            detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                           detail::index_bits<IndexType>(), eos.gamma(), mesh.dx0(), mesh.dx1(),
+                                           detail::index_bits<IndexType>(prim_arrays), eos.gamma(), mesh.dx0(), mesh.dx1(),
                                            mesh.dx2(), dt));
 }
 
@@ -46,7 +46,7 @@ void godunov(
     auto const c = detail::ptrs(cons_arrays);
     detail::check(detail::capi<T>::godunov_dev(exec_space.stream(), &p, &c, detail::ext(prim_arrays, 0),
                                                detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                               detail::index_bits<IndexType>(), eos.gamma(), mesh.dx0(),
+                                               detail::index_bits<IndexType>(prim_arrays), eos.gamma(), mesh.dx0(),
                                                mesh.dx1(), mesh.dx2(), dt_dev, cfl));
 }
 
@@ -66,7 +66,7 @@ void godunov_reference_order(
     auto const c = detail::ptrs(cons_arrays);
     detail::check(detail::capi<T>::godunov_reference_order(
             exec_space.stream(), &p, &c, detail::ext(prim_arrays, 0), detail::ext(prim_arrays, 1),
-            detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(), eos.gamma(), mesh.dx0(), mesh.dx1(),
+            detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(prim_arrays), eos.gamma(), mesh.dx0(), mesh.dx1(),
             mesh.dx2(), dt));
 }
 

@@ -20,7 +20,7 @@ void init_implode(
     auto const p = detail::ptrs(prim_arrays);
     detail::check(detail::capi<T>::init_implode(exec_space.stream(), &p, detail::ext(prim_arrays, 0),
                                                 detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                                detail::index_bits<IndexType>(), mesh.dx0(), mesh.dx1(),
+                                                detail::index_bits<IndexType>(prim_arrays), mesh.dx0(), mesh.dx1(),
                                                 mesh.dx2()));
 }
 
@@ -41,7 +41,7 @@ void init_diagonal(
     auto const p = detail::ptrs(prim_arrays);
     detail::check(detail::capi<T>::init_diagonal(exec_space.stream(), &p, detail::ext(prim_arrays, 0),
                                                  detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                                 detail::index_bits<IndexType>(), mesh.dx0(), mesh.dx1(),
+                                                 detail::index_bits<IndexType>(prim_arrays), mesh.dx0(), mesh.dx1(),
                                                  mesh.dx2(), offset, d_hi, p_hi, d_lo, p_lo));
 }
 
@@ -55,7 +55,7 @@ void init_random(
     detail::check_fields(prim_arrays);
     auto const p = detail::ptrs(prim_arrays);
     detail::check(detail::capi<T>::init_random(exec_space.stream(), &p, static_cast<int64_t>(size(prim_arrays)),
-                                               detail::index_bits<IndexType>(), seed, 0.5, 2.0, 0.5, 2.0, 0.3));
+                                               detail::index_bits<IndexType>(prim_arrays), seed, 0.5, 2.0, 0.5, 2.0, 0.3));
 }
 
 } // namespace eb

@@ -1,8 +1,11 @@
-// Perfect-gas equation of state (euler_operators/perfect_gas.hpp:6-29). The GPU
-// kernels take gamma by value; these members are the host-side formulas.
+// Perfect-gas equation of state (euler_operators/perfect_gas.hpp:6-29). The
+// library's kernels take gamma by value; these members are the same formulas for
+// host code and for device code built with hipcc.
 #pragma once
 
 #include <cmath>
+
+#include "macros.hpp"
 
 namespace eb {
 
@@ -10,25 +13,28 @@ template <class T>
 class PerfectGas
 {
 public:
-    explicit PerfectGas(T const gamma) : m_gamma(gamma) {}
+    EB_FUNCTION explicit PerfectGas(T const gamma) : m_gamma(gamma) {}
 
-    T gamma() const noexcept
+    EB_FUNCTION T gamma() const noexcept
     {
         return m_gamma;
     }
 
-    T speed_of_sound(T const density, T const pressure) const noexcept
+    EB_FUNCTION T speed_of_sound(T const density, T const pressure) const noexcept
     {
+        EB_IEEE_ORDER
         return std::sqrt(m_gamma * pressure / density);
     }
 
-    T internal_energy(T const /*density*/, T const pressure) const noexcept
+    EB_FUNCTION T internal_energy(T const /*density*/, T const pressure) const noexcept
     {
+        EB_IEEE_ORDER
         return pressure / (m_gamma - 1);
     }
 
-    T pressure(T const /*density*/, T const int_e) const noexcept
+    EB_FUNCTION T pressure(T const /*density*/, T const int_e) const noexcept
     {
+        EB_IEEE_ORDER
         return (m_gamma - 1) * int_e;
     }
 

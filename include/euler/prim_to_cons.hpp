@@ -19,7 +19,7 @@ void prim_to_cons(
     auto const c = detail::ptrs(cons_arrays);
     detail::check(detail::capi<T>::prim_to_cons(exec_space.stream(), &p, &c, detail::ext(prim_arrays, 0),
                                                 detail::ext(prim_arrays, 1), detail::ext(prim_arrays, 2),
-                                                detail::index_bits<IndexType>(), eos.gamma()));
+                                                detail::index_bits<IndexType>(prim_arrays), eos.gamma()));
 }
 
 } // namespace eb

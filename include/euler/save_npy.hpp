@@ -30,9 +30,14 @@ char const* npy_descr()
 template <class T>
 void save_npy(std::ostream& os, T const* data, std::array<std::size_t, 3> const& shape)
 {
+    // Every extent is followed by ", " inside the parentheses, as the reference's
+    // writer emits it (simulations/save_npy.cpp:37-46), so the files are byte-identical.
+    std::string dims;
+    for (std::size_t e : shape) {
+        dims += std::to_string(e) + ", ";
+    }
     std::string dict = std::string("{'descr': '") + detail::npy_descr<T>() + "', 'fortran_order': True, 'shape': ("
-                       + std::to_string(shape[0]) + ", " + std::to_string(shape[1]) + ", " + std::to_string(shape[2])
-                       + "), }";
+                       + dims + "), }";
     // magic (6) + version (2) + header length (2) + dict + padding + '\n' is a multiple of 64
     std::size_t const unpadded = 10 + dict.size() + 1;
     dict.append((64 - unpadded % 64) % 64, ' ');

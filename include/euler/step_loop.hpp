@@ -175,7 +175,7 @@ private:
         auto const p = detail::ptrs(m_buf[target]);
         detail::check(detail::capi<T>::ghost_refresh(m_ex.context(), m_ex.stream(), &c, &p, detail::ext(m_cons, 0),
                                                      detail::ext(m_cons, 1), detail::ext(m_cons, 2),
-                                                     detail::index_bits<IndexType>(), m_eos.gamma(), m_mesh.dx0(),
+                                                     detail::index_bits<IndexType>(m_cons), m_eos.gamma(), m_mesh.dx0(),
                                                      m_mesh.dx1(), m_mesh.dx2(), m_dt.data() + 2));
     }
 
@@ -187,7 +187,7 @@ private:
         auto const pout = detail::ptrs(m_buf[dst]);
         detail::check(detail::capi<T>::step_fused(
                 m_ex.context(), stream, &pin, &c, &pout, detail::ext(m_cons, 0), detail::ext(m_cons, 1),
-                detail::ext(m_cons, 2), detail::index_bits<IndexType>(), m_eos.gamma(), m_mesh.dx0(), m_mesh.dx1(),
+                detail::ext(m_cons, 2), detail::index_bits<IndexType>(m_cons), m_eos.gamma(), m_mesh.dx0(), m_mesh.dx1(),
                 m_mesh.dx2(), m_cfl, m_dt.data() + src, m_dt.data() + 2, m_dt.data() + dst, m_time.data()));
     }
 

@@ -22,7 +22,7 @@ T time_step(
     T dt {};
     detail::check(detail::capi<T>::time_step_sync(exec_space.context(), exec_space.stream(), &p,
                                                   detail::ext(prim_arrays, 0), detail::ext(prim_arrays, 1),
-                                                  detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(),
+                                                  detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(prim_arrays),
                                                   eos.gamma(), mesh.dx0(), mesh.dx1(), mesh.dx2(), &dt));
     return dt;
 }
@@ -40,7 +40,7 @@ void time_step(
     auto const p = detail::ptrs(prim_arrays);
     detail::check(detail::capi<T>::time_step(exec_space.context(), exec_space.stream(), &p,
                                              detail::ext(prim_arrays, 0), detail::ext(prim_arrays, 1),
-                                             detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(),
+                                             detail::ext(prim_arrays, 2), detail::index_bits<IndexType>(prim_arrays),
                                              eos.gamma(), mesh.dx0(), mesh.dx1(), mesh.dx2(), dt_dev));
 }
 

@@ -183,7 +183,7 @@ void npy_check()
     std::string const s = os.str();
     bool ok = s.size() == 128 + v.size() * 8 && s.compare(1, 5, "NUMPY") == 0 && s[127] == '\n'
               && s.find("'fortran_order': True") != std::string::npos
-              && s.find("'shape': (2, 3, 4)") != std::string::npos;
+              && s.find("'shape': (2, 3, 4, ), }") != std::string::npos;
     double last = 0;
     std::memcpy(&last, s.data() + s.size() - 8, 8);
     expect(ok && last == 23.5, "save_npy header and payload");

@@ -1,7 +1,9 @@
 """C++ API (include/euler/*.hpp) over the HIP library: the compiled check
-program, and the step-loop driver's .npy output against the CPU oracle's
-reference-order loop (euler_simulation.cpp:50-80), bitwise, for the reference
-loop, the fused loop and the graph-replayed loop."""
+programs (host and device sides of the API, index types past the 32-bit
+kernels' limit), and the step-loop driver's .npy files against the CPU oracle's
+reference-order loop (euler_simulation.cpp:50-80) byte for byte, header
+included, for the reference loop, the fused loop and the graph-replayed loop,
+at the reference driver's own settings (nx 128, nt 200, a file every 10 steps)."""
 from __future__ import annotations
 
 import subprocess
@@ -34,29 +36,75 @@ def test_cpp_api_checks():
     assert "FAIL" not in r.stdout
 
 
-@pytest.mark.parametrize("real", ["f64", "f32"])
-def test_driver_matches_oracle(orc, tmp_path, real):
-    nx, nt, freq, cfl, gamma = 16, 20, 10, 0.49, 1.4
+def test_device_api_checks():
+    """hllc functor, get<Dir>, load / store and subspan views inside kernels."""
+    r = _run(["test_device_api"])
+    assert r.returncode == 0
+    assert "ALL PASS" in r.stdout
+    assert "FAIL" not in r.stdout
+
+
+def test_index_types_past_32_bit_kernels():
+    """int (1290^3 = 2,146,689,000 cells) and unsigned (1300^3 = 2,197,000,000
+    cells, past 2^31) through the C++ operators, which route them to the 64-bit
+    kernels: cells past 2^31 checked against the reference's formulas evaluated
+    on the host through the same public point functions. FP32, about 88 GB of HBM
+    per case, one case at a time."""
+    r = _run(["test_device_api", "--large"], timeout=300)
+    assert r.returncode == 0
+    assert "ALL PASS" in r.stdout
+    assert "FAIL" not in r.stdout
+
+
+def npy_bytes(field: np.ndarray) -> bytes:
+    """The reference writer's file (simulations/save_npy.cpp:34-75) for one field
+    of an oracle state, restated here: header dict with every extent followed by
+    ', ', padded with spaces so that 10 + dict + newline is a multiple of 64, then
+    the layout_left payload. `field` is (n2, n1, n0), so its C order is the
+    Fortran order of [i, j, k]."""
+    n2, n1, n0 = field.shape
+    descr = "<f8" if field.dtype == np.float64 else "<f4"
+    shape = "".join(f"{e}, " for e in (n0, n1, n2))
+    d = "{'descr': '" + descr + "', 'fortran_order': True, 'shape': (" + shape + "), }"
+    total = 10 + len(d) + 1
+    d += " " * ((total + 63) // 64 * 64 - total) + "\n"
+    head = b"\x93NUMPY\x01\x00" + len(d).to_bytes(2, "little") + d.encode("latin1")
+    return head + np.ascontiguousarray(field).tobytes()
+
+
+@pytest.mark.parametrize("real,nx,nt,index", [("f64", 128, 200, "int"), ("f32", 16, 20, "unsigned"),
+                                              ("f64", 16, 20, "long")])
+def test_driver_matches_oracle(orc, tmp_path, real, nx, nt, index):
+    """The reference driver's settings (nx 128, nt 200, CFL 0.49, a file every 10
+    steps, FP64, int) and smaller FP32 / unsigned and int64 runs: every file equals
+    the oracle's byte for byte."""
+    freq, cfl, gamma = 10, 0.49, 1.4
     dtype = np.float64 if real == "f64" else np.float32
-    outs = {}
-    for loop in ("reference", "fused", "graph"):
-        d = tmp_path / loop
-        d.mkdir()
-        r = _run(["euler_simulation", "--nx", nx, "--nt", nt, "--output-freq", freq, "--cfl", cfl, "--real", real,
-                  "--loop", loop, "--out", d])
-        assert r.returncode == 0
-        outs[loop] = [np.load(d / f"test_{it:010d}.npy") for it in (10, 20)]
     dx = 1.0 / nx
     na = nx + 2
     prim = orc.init_implode(na, na, na, dx, dx, dx, dtype=dtype)
     cons = orc.prim_to_cons(prim, gamma)
-    for s, it in enumerate((10, 20)):
-        orc.simulate(prim, cons, gamma, dx, dx, dx, cfl, 10)
-        ref = prim[1].transpose(2, 1, 0)  # (n2, n1, n0) -> [i, j, k]
-        for loop, arrs in outs.items():
-            got = arrs[s]
-            assert got.dtype == dtype and got.shape == (na, na, na)
-            assert np.array_equal(got, ref), (loop, it)
+    expected = {}
+    for it in range(freq, nt + 1, freq):
+        orc.simulate(prim, cons, gamma, dx, dx, dx, cfl, freq)
+        expected[it] = npy_bytes(prim[1])
+    for loop in ("reference", "fused", "graph"):
+        d = tmp_path / loop
+        d.mkdir()
+        r = _run(["euler_simulation", "--nx", nx, "--nt", nt, "--output-freq", freq, "--cfl", cfl, "--real", real,
+                  "--index", index, "--loop", loop, "--out", d])
+        assert r.returncode == 0
+        for it, want in expected.items():
+            path = d / f"test_{it:010d}.npy"
+            got = path.read_bytes()
+            assert got[:128] == want[:128], (loop, it, got[:128])
+            assert got == want, (loop, it)
+            path.unlink()
+    # the expected bytes are a valid .npy of the pressure indexed [i, j, k]
+    import io
+
+    back = np.load(io.BytesIO(expected[nt]))
+    assert back.shape == (na, na, na) and np.array_equal(back, prim[1].transpose(2, 1, 0))
 
 
 def test_benchmarks_write_google_benchmark_json(tmp_path):
