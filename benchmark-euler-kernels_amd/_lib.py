@@ -86,6 +86,7 @@ def _signatures():
         "eb_batch_step": [_vp, _vp, _int],
         "eb_batch_step_per_instance": [_vp, _vp, _int],
         "eb_batch_autotune": [_vp, _vp, _int, _int, _P(_f32)],
+        "eb_time_step_noalloc_f64": [_vp, _P(PrimPtrs), _i64, _i64, _i64, _int, _f64, _f64, _f64, _f64, _vp],
     }
     for suf, real in (("f64", _f64), ("f32", _f32)):
         ext = [_i64, _i64, _i64, _int]

@@ -335,7 +335,7 @@ int batch_start(eb_batch* b, hipStream_t s)
         prim_c<T> const pa {in.buf[0].d, in.buf[0].p, in.buf[0].ux0, in.buf[0].ux1, in.buf[0].ux2};
         prim_c<T> const pb {in.buf[1].d, in.buf[1].p, in.buf[1].ux0, in.buf[1].ux1, in.buf[1].ux2};
         cons_c<T> const c {in.cons.d, in.cons.e, in.cons.mx0, in.cons.mx1, in.cons.mx2};
-        int st = run_time_step<T, I>(&ctx, s, &pa, in.n0, in.n1, in.n2, in.gamma, in.dx0, in.dx1, in.dx2, in.dt);
+        int st = run_time_step<T, I>(ctx.scratch, s, &pa, in.n0, in.n1, in.n2, in.gamma, in.dx0, in.dx1, in.dx2, in.dt);
         if (st == EB_OK) {
             st = run_ghost_refresh<T, I>(&ctx, s, &c, &pb, in.n0, in.n1, in.n2, in.gamma, in.dx0, in.dx1, in.dx2,
                                          in.dt + 2);

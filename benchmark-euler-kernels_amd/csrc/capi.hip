@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <limits>
+#include <mutex>
 #include <string>
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,14 @@ struct eb_context
     double* dt_slot;         // device landing slot for the blocking time step
     double* partials;        // per-block (sum, compensation) pairs of the FP64 diagnostics
 };
+
+// Reduction words of eb_time_step_noalloc_f64, which allocates nothing: one
+// record of the single-pass grid minimum (the running minimum's ordered encoding
+// and the count of blocks that have contributed, 16 bytes), in a device array that
+// every GPU the library runs on holds until the process ends. On the host the
+// name `scratch` is not a device address: host code uses the per-device address
+// from hipGetSymbolAddress (noalloc_scratch below) for kernel arguments.
+static __device__ __align__(256) eb::MinScratch scratch[1];
 
 namespace {
 
@@ -171,9 +180,15 @@ int run_prim_to_cons(hipStream_t s, PS const* prim, CS const* cons, int64_t n0, 
 }
 
 template <class T>
+eb::DtSink<T> plain_sink(eb::MinScratch* scratch, T* dt_out)
+{
+    return eb::DtSink<T> {scratch, dt_out, nullptr, nullptr, nullptr, T(0)};
+}
+
+template <class T>
 eb::DtSink<T> plain_sink(eb_context* ctx, T* dt_out)
 {
-    return eb::DtSink<T> {ctx->scratch, dt_out, nullptr, nullptr, nullptr, T(0)};
+    return plain_sink<T>(ctx->scratch, dt_out);
 }
 
 template <class T>
@@ -185,7 +200,7 @@ bool aligned16(eb::PrimPtrs<T const> const& p)
 
 template <class T, class I, class PS>
 int run_time_step(
-        eb_context* ctx,
+        eb::MinScratch* scratch,
         hipStream_t s,
         PS const* prim,
         int64_t n0,
@@ -207,10 +222,10 @@ int run_time_step(
     if (aligned16(pc)) {
         // sized in groups of 16 bytes, so that each thread has four groups in flight
         hipLaunchKernelGGL((eb::k_time_step_vec<T, I>), dim3(reduction_grid(n / (16 / int64_t(sizeof(T))))), dim3(256),
-                           0, s, pc, static_cast<I>(n), gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+                           0, s, pc, static_cast<I>(n), gamma, invdx0, invdx1, invdx2, plain_sink<T>(scratch, dt_dev));
     } else {
         hipLaunchKernelGGL((eb::k_time_step<T, I>), dim3(reduction_grid(n)), dim3(256), 0, s, pc, static_cast<I>(n),
-                           gamma, invdx0, invdx1, invdx2, plain_sink<T>(ctx, dt_dev));
+                           gamma, invdx0, invdx1, invdx2, plain_sink<T>(scratch, dt_dev));
     }
     return check_launch("time_step");
 }
@@ -588,6 +603,31 @@ int run_boundary_flux(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0
     return check_launch("boundary_flux");
 }
 
+// Device address of `scratch` on the current device, looked up once per device
+// (hipGetSymbolAddress allocates nothing) and kept for later calls.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+eb::MinScratch* g_noalloc_scratch[kMaxDevices] = {};
+
+int noalloc_scratch(eb::MinScratch** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        return fail(EB_ERR_HIP, "no current HIP device (or device index >= 64)");
+    }
+    std::lock_guard<std::mutex> const lock(g_noalloc_mutex);
+    if (g_noalloc_scratch[dev] == nullptr) {
+        void* p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            return fail(EB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_noalloc_scratch[dev] = static_cast<eb::MinScratch*>(p);
+    }
+    *out = g_noalloc_scratch[dev];
+    return EB_OK;
+}
+
 } // namespace
 
 // ======================================================================= ABI
@@ -637,6 +677,31 @@ int eb_context_destroy(eb_context* ctx)
     int const rc = ok ? EB_OK : fail(EB_ERR_HIP, "hipFree failed");
     delete ctx;
     return rc;
+}
+
+int eb_time_step_noalloc_f64(hipStream_t stream, const eb_prim_f64* prim, int64_t n0, int64_t n1, int64_t n2,
+                             int index_bits, double gamma, double dx0, double dx1, double dx2, double* dt_dev)
+{
+    int rc = check_extents(n0, n1, n2, index_bits);
+    if (rc != EB_OK) {
+        return rc;
+    }
+    if (dt_dev == nullptr || (n0 * n1 * n2 > 0 && prim_null(prim))) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    eb::MinScratch* words = nullptr;
+    if ((rc = noalloc_scratch(&words)) != EB_OK) {
+        return rc;
+    }
+    // The record starts from its initial state on every call, set by a fill kernel
+    // (not a memset, so that the call can be captured in a graph).
+    hipLaunchKernelGGL(eb::min_scratch_init, dim3(1), dim3(64), 0, stream, words, 1);
+    if ((rc = check_launch("time_step_noalloc: scratch init")) != EB_OK) {
+        return rc;
+    }
+    return index_bits == 32
+                   ? run_time_step<double, int32_t>(words, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)
+                   : run_time_step<double, int64_t>(words, stream, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev);
 }
 
 const char* eb_status_string(int status)
@@ -741,8 +806,8 @@ int abi_time_step(eb_context* ctx, hipStream_t s, PS const* prim, int64_t n0, in
     if (ctx == nullptr || dt_dev == nullptr || (n0 * n1 * n2 > 0 && prim_null(prim))) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
     }
-    return bits == 32 ? run_time_step<T, int32_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)
-                      : run_time_step<T, int64_t>(ctx, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev);
+    return bits == 32 ? run_time_step<T, int32_t>(ctx->scratch, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev)
+                      : run_time_step<T, int64_t>(ctx->scratch, s, prim, n0, n1, n2, gamma, dx0, dx1, dx2, dt_dev);
 }
 
 template <class T, class PS>

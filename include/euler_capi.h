@@ -111,6 +111,18 @@ int eb_time_step_sync_f64(eb_context* ctx, hipStream_t stream, const eb_prim_f64
 int eb_time_step_sync_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32* prim,
                           int64_t n0, int64_t n1, int64_t n2, int index_bits, float gamma,
                           float dx0, float dx1, float dx2, float* dt_host);
+/* eb_time_step_f64 without a context, for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): no
+ * reference counterpart. It allocates nothing. Its reduction record (16 bytes: the
+ * running minimum and a block count) lives in a static device array that every
+ * GPU the library runs on holds until the process ends; the record's size does
+ * not grow with the extents, so these are limited only as for eb_time_step_f64.
+ * The record is reset by a small kernel at the start of every call. Calls that
+ * overlap in time on one GPU (two streams, or two host threads) share the record
+ * and race: serialise them. Same result as eb_time_step_f64, raw dt in dt_dev[0]. */
+int eb_time_step_noalloc_f64(hipStream_t stream, const eb_prim_f64* prim,
+                             int64_t n0, int64_t n1, int64_t n2, int index_bits, double gamma,
+                             double dx0, double dx1, double dx2, double* dt_dev);
 
 /* ---- godunov.hpp:15-92 (hllc, PerfectGas, UniformMesh3d) ----------------
  * Updates cons in place on interior cells from the (const) primitives with a
