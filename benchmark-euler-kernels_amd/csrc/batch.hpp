@@ -106,11 +106,12 @@ __attribute__((amdgpu_waves_per_eu(Layout == 2 ? 2 : 1))) void k_batch_step(
             = gridDim.x / nq + ((static_cast<int>(gridDim.x % nq) > q) ? 1u : 0u);
     bool const tune = tune_dt != nullptr;
 
-    // The tile is fetched by the first lane of wave 1 and the time-step commit
-    // below is done by the first lane of wave 0: two different lanes, so that the
-    // compiler never merges the two single-lane regions across the loop back edge
-    // (a merged region would put the loop-head barrier on two divergent paths of
-    // wave 0, i.e. make that wave reach it twice).
+    // Invariant: no single-lane region may span the loop-head barrier (a region
+    // merged across the back edge would put that barrier on two divergent paths of
+    // one wave, i.e. make the wave reach it twice, and hang the block). The time-step
+    // commit at the end of an iteration is therefore closed by a barrier of its own,
+    // and, as a second guard, the tile is fetched by another lane (the first lane of
+    // wave 1) than the one that commits (the first lane of wave 0).
     int const fetch_lane = 64;
     for (;;) {
         if (lid == fetch_lane) {
@@ -162,6 +163,7 @@ __attribute__((amdgpu_waves_per_eu(Layout == 2 ? 2 : 1))) void k_batch_step(
                                   tune ? nullptr : in.time, dt_in, in.cfl};
             grid_min_commit(sink, bm, static_cast<unsigned int>(in.ntiles));
         }
+        __syncthreads(); // closes the commit region inside this iteration
     }
     if (lid == 0) {
         unsigned int const d = atomicAdd(&queues[q].done, 1u);

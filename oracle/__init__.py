@@ -69,14 +69,14 @@ class Oracle:
         getattr(self.lib, f"oracle_hllc_{suf}")(direction, real(gamma), qL.ctypes.data, qR.ctypes.data, out.ctypes.data)
         return out
 
-    def cons_to_prim(self, cons: np.ndarray, gamma: float) -> np.ndarray:
-        prim = np.zeros_like(cons)
+    def cons_to_prim(self, cons: np.ndarray, gamma: float, out: np.ndarray | None = None) -> np.ndarray:
+        prim = np.zeros_like(cons) if out is None else out
         suf, real = self._suf(cons)
         getattr(self.lib, f"oracle_cons_to_prim_{suf}")(_fields(cons), _fields(prim), real(gamma), cons[0].size)
         return prim
 
-    def prim_to_cons(self, prim: np.ndarray, gamma: float) -> np.ndarray:
-        cons = np.zeros_like(prim)
+    def prim_to_cons(self, prim: np.ndarray, gamma: float, out: np.ndarray | None = None) -> np.ndarray:
+        cons = np.zeros_like(prim) if out is None else out
         suf, real = self._suf(prim)
         getattr(self.lib, f"oracle_prim_to_cons_{suf}")(_fields(prim), _fields(cons), real(gamma), prim[0].size)
         return cons
