@@ -48,6 +48,11 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (HBM counter calibration)")
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
+                    help="profiling aid: only the batched ensemble, STEPS graph-replayed steps after autotuning")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="with --quick: also run prim_to_cons on the same arrays (HBM counter calibration: "
+                         "one 8-byte load and one 8-byte store per element, no reuse)")
     return ap.parse_args()
 
 
@@ -286,6 +291,21 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
     }
 
 
+def batch_profile(torch, eb, ex, timer, nsteps: int, count: int = 256):
+    """The measured ensemble stepped nsteps times after autotuning (for counters)."""
+    prims, conss, eoss, meshes, cfls, params = eb.batch.random_ensemble(ex, count, seed=1234, dtype=torch.float32,
+                                                                        index_bits=32, cfl=0.5)
+    ens = eb.batch.Ensemble(ex, prims, conss, eoss, meshes, cfls)
+    ens.start()
+    ens.step()
+    ens.autotune(reps=3)
+    ens.run(3, use_graph=True)
+    ms = timer.run(lambda: ens.run(nsteps, use_graph=True), 1) / nsteps
+    tiles = ens.tiles()
+    ens.close()
+    return {"batched_graph_ms_per_step": round(ms, 4), "tiles": tiles, "steps": nsteps}
+
+
 def fp32_report(torch, eb, ex, n: int = 512, nsteps: int = 1000, cfl: float = 0.5):
     """Diagonal implosion (x+y+z > 0.45: rho = p = 1, else rho = 0.125, p = 0.14;
     u = 0; gamma 1.4; unit cube) at n^3, nsteps steps, FP32 state and arithmetic
@@ -405,6 +425,11 @@ def main():
     torch.cuda.set_device(dev)
     ex = eb.ExecSpace(dev)
     timer = Timer(torch, ex.stream)
+    if args.batch_only:
+        log(f"batched ensemble only: {args.batch_only} graph-replayed steps")
+        print(json.dumps(batch_profile(torch, eb, ex, timer, args.batch_only)), flush=True)
+        ex.close()
+        return
     n = args.n
     na = n + 2
     n_int = n**3
@@ -442,6 +467,11 @@ def main():
     achieved = bytes_godunov / (ms * 1e-3) / 1e9
     value = n_int / (ms_wall * 1e-3) / 1e6
     log(f"godunov FP64: {ms:.3f} ms/launch (events), {ms_wall:.3f} ms/step (wall), {achieved:.0f} GB/s")
+
+    if args.traffic_calibration:
+        for _ in range(3):
+            eb.prim_to_cons(ex, prim, cons, eos)
+        torch.cuda.synchronize()
 
     extra = {
         "godunov_f64_ms_event": round(ms, 4),

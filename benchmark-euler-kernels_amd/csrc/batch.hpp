@@ -88,9 +88,9 @@ __device__ __forceinline__ int wave_uniform(int v)
 template <class T, class I, int BY, int Layout>
 __global__ __launch_bounds__(Layout == 2 ? 32 * BY : 64 * BY)
 __attribute__((amdgpu_waves_per_eu(Layout == 2 ? 2 : 1))) void k_batch_step(
-        BatchInstance<T> const* insts,
-        BatchTile const* tiles,
-        int const* ranges,
+        BatchInstance<T> const* __restrict__ insts,
+        BatchTile const* __restrict__ tiles,
+        int const* __restrict__ ranges,
         int nq,
         BatchQueue* queues,
         int parity,

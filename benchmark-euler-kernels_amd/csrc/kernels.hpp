@@ -363,6 +363,14 @@ __device__ __forceinline__ void uniform_opaque(I& v)
 #endif
 }
 
+// Pointer into the global address space. Every field array is device memory, so
+// accesses through it are global loads and stores even where the pointer was read
+// from memory (the batched ensemble's instance descriptors), where the compiler
+// would otherwise emit flat accesses, which also count against the LDS wait
+// counter and so make every LDS barrier wait for the global traffic in flight.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T*;
+
 // Plane-relative addressing: a wave-uniform plane base pointer (scalar registers,
 // index type I, so 64-bit extents are honoured) plus a per-lane 32-bit byte offset
 // inside the plane. This is the saddr + voffset form of global loads/stores: one
@@ -370,13 +378,13 @@ __device__ __forceinline__ void uniform_opaque(I& v)
 template <class T>
 __device__ __forceinline__ T ld(T const* plane, uint32_t byte_off)
 {
-    return *reinterpret_cast<T const*>(reinterpret_cast<char const*>(plane) + byte_off);
+    return *(global_ptr<T const>)(reinterpret_cast<char const*>(plane) + byte_off);
 }
 
 template <class T>
 __device__ __forceinline__ void st(T* plane, uint32_t byte_off, T v)
 {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + byte_off) = v;
+    *(global_ptr<T>)(reinterpret_cast<char*>(plane) + byte_off) = v;
 }
 
 template <class T, class I>
