@@ -44,6 +44,17 @@ def test_device_api_checks():
     assert "FAIL" not in r.stdout
 
 
+def test_plug_points():
+    """include/euler/plugins.hpp: the generic kernels with the reference's solver,
+    EOS and mesh equal the library's kernels bitwise; a caller's Rusanov solver,
+    stiffened-gas EOS and mesh type run on the GPU and equal a host evaluation of
+    the reference's loops through the same functors, bitwise (FP64 and FP32)."""
+    r = _run(["test_plugins"])
+    assert r.returncode == 0
+    assert "ALL PASS" in r.stdout
+    assert "FAIL" not in r.stdout
+
+
 def test_index_types_past_32_bit_kernels():
     """int (1290^3 = 2,146,689,000 cells) and unsigned (1300^3 = 2,197,000,000
     cells, past 2^31) through the C++ operators, which route them to the 64-bit
