@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="profiling aid: only the batched ensemble, STEPS graph-replayed steps after autotuning")
+    ap.add_argument("--traffic-record", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
+                    help="write the profiled-traffic record from two rocprofv3 --pmc counter CSVs and exit")
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="with --quick: also run prim_to_cons on the same arrays (HBM counter calibration: "
                          "one 8-byte load and one 8-byte store per element, no reuse)")
@@ -387,15 +389,79 @@ def fp32_report(torch, eb, ex, n: int = 512, nsteps: int = 1000, cfl: float = 0.
     }
 
 
+TRAFFIC_RECORD = ROOT / "profiles" / "godunov_f64_512_traffic.json"
+
+
+def source_digest() -> str:
+    """SHA-256 of the library's kernel and C ABI sources: a traffic figure measured
+    on other sources is not reported as this build's."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = ROOT / "benchmark-euler-kernels_amd" / "csrc"
+    for p in sorted(list(csrc.glob("*.hpp")) + list(csrc.glob("*.hip")) + [ROOT / "include" / "euler_capi.h"]):
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
+    return h.hexdigest()
+
+
 def profiled_traffic(kernel: str, n: int):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated) for this kernel and workload, from the committed profile
-    summary; None when the live configuration was not profiled."""
-    path = ROOT / "profiles" / "godunov_f64_512_traffic.json"
-    if n != 512 or not path.exists():
+    calibrated on prim_to_cons) for this kernel and workload, from the committed
+    record; None when the record is for another kernel, size or source tree."""
+    if n != 512 or not TRAFFIC_RECORD.exists():
         return None
-    rec = json.loads(path.read_text())
-    return rec["traffic_bytes_per_launch"] if rec.get("kernel") == kernel else None
+    rec = json.loads(TRAFFIC_RECORD.read_text())
+    if rec.get("kernel") != kernel or rec.get("library_source_sha256") != source_digest():
+        log("profiled traffic record is for another kernel or source tree: traffic not reported")
+        return None
+    return rec["traffic_bytes_per_launch"]
+
+
+def write_traffic_record(fetch_csv: str, write_csv: str, n: int = 512) -> dict:
+    """Traffic record from two rocprofv3 --pmc runs of `bench.py --quick
+    --traffic-calibration` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch). Each
+    counter is scaled by what it reads for k_prim_to_cons<double, long> on the same
+    arrays, which loads and stores each of 5 fields once, 8 bytes per element."""
+    import csv
+    import statistics
+
+    def per_kernel(path, counter):
+        out = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+        return out
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    na = n + 2
+    calib_bytes = 5 * 8 * na**3
+
+    def pick(d, pat):
+        keys = [k for k in d if pat in k]
+        assert len(keys) == 1, (pat, keys)
+        return keys[0], statistics.median(d[keys[0]])
+
+    gname, gf = pick(fetch, "k_godunov_pair<double, long")
+    _, gw = pick(write, "k_godunov_pair<double, long")
+    _, cf = pick(fetch, "k_prim_to_cons<double, long>")
+    _, cw = pick(write, "k_prim_to_cons<double, long>")
+    fs, ws = calib_bytes / cf, calib_bytes / cw
+    kernel = "eb::" + gname.split("eb::", 1)[1].split("(", 1)[0]
+    rec = {
+        "kernel": kernel,
+        "workload": f"godunov FP64 {n}^3 interior, int64 indexing",
+        "traffic_bytes_per_launch": gf * fs + gw * ws,
+        "fetch_bytes": gf * fs,
+        "write_bytes": gw * ws,
+        "algorithmic_bytes": 8 * (5 * na**3 + 10 * n**3),
+        "fetch_scale": fs,
+        "write_scale": ws,
+        "calibration": "k_prim_to_cons<double, long> on the same (n+2)^3 arrays: 5 fields read and written once",
+        "library_source_sha256": source_digest(),
+    }
+    TRAFFIC_RECORD.write_text(json.dumps(rec, indent=1) + "\n")
+    return rec
 
 
 def kernel_name(eb) -> str:
@@ -409,6 +475,9 @@ def kernel_name(eb) -> str:
 
 def main():
     args = parse()
+    if args.traffic_record:
+        print(json.dumps(write_traffic_record(*args.traffic_record)))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(the multi-GPU halo design is documented, not built)"}))
