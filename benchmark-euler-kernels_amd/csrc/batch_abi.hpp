@@ -46,10 +46,11 @@ using inst_c = std::conditional_t<std::is_same_v<T, double>, eb_batch_instance_f
 
 constexpr int kBatchQueues = 8; // one per XCD
 
-bool batch_by_supported(int layout, int by)
+// Linear tiles of 1024 cells stage 147 KB of LDS in FP32; FP64 would not fit.
+bool batch_by_supported(int layout, int by, int real_bits)
 {
     return layout == 0   ? (by == 4 || by == 8 || by == 16)
-           : layout == 1 ? (by == 4 || by == 8)
+           : layout == 1 ? (by == 4 || by == 8 || (by == 16 && real_bits == 32))
                          : (layout == 2 && (by == 8 || by == 16));
 }
 
@@ -91,6 +92,11 @@ template <class T, class I>
 int batch_resident(int layout, int by, int* out)
 {
     if (layout == 1) {
+        if constexpr (sizeof(T) == 4) {
+            if (by == 16) {
+                return batch_resident_blocks<T, I, 16, 1>(out);
+            }
+        }
         return by == 4 ? batch_resident_blocks<T, I, 4, 1>(out) : batch_resident_blocks<T, I, 8, 1>(out);
     }
     if (layout == 2) {
@@ -386,6 +392,12 @@ int batch_launch(eb_batch* b, hipStream_t s, int parity, bool tune)
             EB_BATCH_LAUNCH_PAIR(16);
         }
     } else if (b->layout == 1) {
+        if constexpr (sizeof(T) == 4) {
+            if (b->by == 16) {
+                EB_BATCH_LAUNCH(16, 1);
+                return check_launch("batch_step");
+            }
+        }
         if (b->by == 4) {
             EB_BATCH_LAUNCH(4, 1);
         } else {
@@ -431,7 +443,7 @@ int batch_per_instance(eb_batch* b, hipStream_t s, int parity)
 template <class T, class I>
 int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best_ms)
 {
-    static constexpr int kShapes[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 4}, {1, 8}, {2, 8}, {2, 16}}; // (layout, by)
+    static constexpr int kShapes[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 4}, {1, 8}, {1, 16}, {2, 8}, {2, 16}}; // (layout, by)
     static constexpr int kKc[] = {0, 64, 32, 16, 8};
     int64_t max_n0 = 0;
     for (int i = 0; i < b->ninst; ++i) {
@@ -450,7 +462,7 @@ int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best
     for (auto const& shape : kShapes) {
         int const layout = shape[0];
         int const by = shape[1];
-        if (layout == 1 && max_n0 > 64 * by) {
+        if ((layout == 1 && max_n0 > 64 * by) || !batch_by_supported(layout, by, 8 * int(sizeof(T)))) {
             continue;
         }
         for (int kc : kKc) {
@@ -560,9 +572,9 @@ int eb_batch_set_tiles(eb_batch* batch, int layout, int by, int kchunk)
     if (batch == nullptr || kchunk < 0) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null batch or negative kchunk");
     }
-    if (!batch_by_supported(layout, by)) {
-        return fail(EB_ERR_UNSUPPORTED,
-                    "tiles: layout 0 with by 4, 8 or 16; layout 1 with by 4 or 8; layout 2 with by 8 or 16");
+    if (!batch_by_supported(layout, by, batch->real_bits)) {
+        return fail(EB_ERR_UNSUPPORTED, "tiles: layout 0 with by 4, 8 or 16; layout 1 with by 4 or 8 (or 16 in "
+                                        "FP32); layout 2 with by 8 or 16");
     }
     int const keep_layout = batch->layout;
     int const keep_by = batch->by;

@@ -320,8 +320,8 @@ int eb_batch_create_f64(eb_batch** batch, const eb_batch_instance_f64* instances
 int eb_batch_create_f32(eb_batch** batch, const eb_batch_instance_f32* instances, int ninstances, int index_bits);
 int eb_batch_destroy(eb_batch* batch);
 /* Tile layout 0: 64 x by columns of cells, by in {4, 8, 16}; layout 1: runs of
- * 64*by consecutive cells of a plane, by in {4, 8}, every instance with
- * n0 <= 64*by; layout 2: 64 x by columns with two rows of cells per lane, by in
+ * 64*by consecutive cells of a plane, by in {4, 8} (also 16 in FP32), every
+ * instance with n0 <= 64*by; layout 2: 64 x by columns with two rows of cells per lane, by in
  * {8, 16}. kchunk planes per tile (0: whole columns). Default (0, 8, 32).
  * eb_batch_autotune picks all three by timing. On failure the previous layout
  * stays in place. */

@@ -95,6 +95,20 @@ def test_batch_tile_layouts_are_bitwise_equal(eb, ex, orc, tiles):
     ens.close()
 
 
+@pytest.mark.parametrize("index_bits,kchunk", [(32, 0), (64, 3)])
+def test_batch_fp32_linear_1024_cell_tiles(eb, ex, orc, index_bits, kchunk):
+    nsteps = 5
+    print(f"batch fp32 linear 1024-cell tiles index {index_bits} kchunk {kchunk}", flush=True)
+    inst = _instances(eb, orc, np.float32, index_bits)
+    refs = _reference(orc, inst, nsteps)
+    ens, P, C = _ensemble(eb, ex, inst, index_bits)
+    ens.set_tiles(16, kchunk, layout=1)
+    assert ens.tiles()["by"] == 16 and ens.tiles()["layout"] == 1
+    ens.run(nsteps)
+    _check(ens, P, C, refs, np.float32, nsteps, inst)
+    ens.close()
+
+
 def test_batch_per_instance_launches_and_graphs(eb, ex, orc):
     nsteps = 9
     inst = _instances(eb, orc, np.float32, 32)
@@ -137,7 +151,7 @@ def test_batch_error_paths(eb, ex, orc):
     with pytest.raises(EulerError):
         ens.set_tiles(8, -1)
     with pytest.raises(EulerError):
-        ens.set_tiles(16, 8, layout=1)
+        ens.set_tiles(16, 8, layout=1)  # 1024-cell linear tiles: FP32 only
     with pytest.raises(EulerError):
         ens.set_tiles(4, 8, layout=2)  # paired rows: 8 or 16 rows per tile
     with pytest.raises(EulerError):
