@@ -244,6 +244,10 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
     prims, conss, eoss, meshes, cfls, params = eb.batch.random_ensemble(ex, count, seed=1234, dtype=torch.float32,
                                                                         index_bits=32, cfl=0.5)
     cells = sum(p["n"][0] * p["n"][1] * p["n"][2] for p in params)
+    arr_cells = sum((p["n"][0] + 2) * (p["n"][1] + 2) * (p["n"][2] + 2) for p in params)
+    # fused step per instance: primitives read on every array cell, conservatives read
+    # and written and next primitives written on the interior cells (FP32)
+    step_bytes = 4 * (5 * arr_cells + 15 * cells)
     ens = eb.batch.Ensemble(ex, prims, conss, eoss, meshes, cfls)
     ens.start()
     ens.step()
@@ -277,6 +281,9 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
         "autotune_seconds": round(tune_s, 2),
         "batched_graph_ms_per_step": round(ms_graph, 4),
         "batched_graph_Mcell_updates_per_s": rate(ms_graph),
+        "algorithmic_bytes_per_step": step_bytes,
+        "batched_graph_GBps_algorithmic": round(step_bytes / ms_graph / 1e6, 1),
+        "batched_graph_hbm_frac": round(step_bytes / ms_graph / 1e6 / 8000.0, 3),
         "batched_eager_ms_per_step": round(ms_eager, 4),
         "per_instance_eager_ms_per_step": round(ms_pi_eager, 4),
         "per_instance_eager_Mcell_updates_per_s": rate(ms_pi_eager),
