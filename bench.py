@@ -235,6 +235,29 @@ def adjoint_leg(torch, eb, ex, timer, n: int = 128, nsteps: int = 200, k: int = 
     }
 
 
+def stream_probe(torch, eb, ex, timer, n_arr: int, dtype):
+    """eb_stream_probe_*: b.f += a.f for the five fields over n_arr cells (five
+    fields read from one array set, five read and written in place in another, one
+    access of the element size each), the plain flux kernel's HBM traffic per cell
+    with no tiles, halos or barriers. Returns (ms per launch, GB/s)."""
+    import ctypes
+
+    from benchmark_euler_kernels_amd import operators as ebo
+    from benchmark_euler_kernels_amd._lib import call
+
+    suf = "f64" if dtype == torch.float64 else "f32"
+    a = eb.create_prim_arrays_1d(ex, n_arr, dtype)
+    b = eb.create_cons_arrays_1d(ex, n_arr, dtype)
+    for t in (*a.tensors(), *b.tensors()):
+        t.zero_()
+    pa, pb = ebo.prim_ptrs(a), ebo.cons_ptrs(b)
+    fn = lambda: call(f"eb_stream_probe_{suf}", ex.handle, ctypes.byref(pa), ctypes.byref(pb), n_arr)  # noqa: E731
+    timer.run(fn, 3)
+    ms = timer.run(fn, 10)
+    elem = 8 if dtype == torch.float64 else 4
+    return ms, 15 * elem * n_arr / ms / 1e6
+
+
 def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_steps: int = 50):
     """The measured ensemble: `count` diagonal implosions (instance i drawn from
     seed 1234+i: interior extents U{32..160} per axis, gamma ~ U[1.2,1.67],
@@ -571,6 +594,13 @@ def main():
                                               "GBps": round(probe, 1),
                                               "godunov_f64_frac_of_probe": round(achieved / probe, 4)}
         del pa, pb, pc
+        # The flux kernel's own traffic per cell streamed with no arithmetic, tiles,
+        # halos or barriers: what plain streaming of that traffic reaches on this part.
+        ms_sprobe, sprobe_gbs = stream_probe(torch, eb, ex, timer, n_arr, torch.float64)
+        extra["stream_probe_godunov_traffic"] = {
+            "kernel": "eb_stream_probe_f64: b.f += a.f for 5 fields over 514^3 FP64 arrays (8-byte accesses)",
+            "ms": round(ms_sprobe, 4), "GBps": round(sprobe_gbs, 1), "hbm_frac": round(sprobe_gbs / HBM_PEAK_GBS, 4),
+            "godunov_f64_frac_of_probe": round(achieved / sprobe_gbs, 4)}
         # Reference-order kernel (six solves per cell), same state.
         ms_ro = timer.run(lambda: eb.godunov(ex, prim, cons, eos, mesh, hllc, dt, reference_order=True), reps)
         extra["godunov_reference_order_f64_ms"] = round(ms_ro, 4)
@@ -615,6 +645,13 @@ def main():
             "GBps_algorithmic": round(4 * (5 * n_arr + 10 * n_int) / ms32 / 1e6, 1),
             "speedup_vs_f64": round(ms / ms32, 3),
         }
+        ms_sp32, sp32_gbs = stream_probe(torch, eb, ex, timer, n_arr, torch.float32)
+        g32 = 4 * (5 * n_arr + 10 * n_int) / ms32 / 1e6
+        extra["stream_probe_godunov_traffic_f32"] = {
+            "kernel": "eb_stream_probe_f32: b.f += a.f for 5 fields over 514^3 FP32 arrays (4-byte accesses)",
+            "ms": round(ms_sp32, 4), "GBps": round(sp32_gbs, 1), "hbm_frac": round(sp32_gbs / HBM_PEAK_GBS, 4),
+            "godunov_f32_frac_of_probe": round(g32 / sp32_gbs, 4),
+            "probe_speedup_f32_vs_f64": round(ms_sprobe / ms_sp32, 3)}
         del prim32, cons32
         torch.cuda.empty_cache()
 
@@ -682,6 +719,9 @@ def main():
             "kernel": kernel_name(eb),
             "bytes_per_launch": bytes_godunov,
             "avg_launch_ms": round(ms, 4),
+            # the same traffic per cell streamed with no arithmetic, tiles or barriers
+            # (extra.stream_probe_godunov_traffic; not run with --quick)
+            "stream_probe_GBps": extra.get("stream_probe_godunov_traffic", {}).get("GBps"),
         },
         "cpu_baseline": cpu,
         "extra": extra,

@@ -106,6 +106,7 @@ def _signatures():
         )
         sig[f"eb_init_implode_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 3
         sig[f"eb_fill_{suf}"] = [_vp, _vp, ctypes.c_int64, real]
+        sig[f"eb_stream_probe_{suf}"] = [_vp, _P(PrimPtrs), _P(ConsPtrs), ctypes.c_int64]
         sig[f"eb_init_diagonal_{suf}"] = [_vp, _P(PrimPtrs)] + ext + [real] * 8
         sig[f"eb_init_random_{suf}"] = [_vp, _P(PrimPtrs), _i64, _int, _u64] + [_f64] * 5
         sig[f"eb_godunov_vjp_{suf}"] = (

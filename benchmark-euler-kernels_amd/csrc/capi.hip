@@ -34,12 +34,30 @@ namespace {
 
 thread_local std::string g_last_error;
 
-// Godunov launch configuration (process-wide): kernel variant, tile height, planes
-// per block. Variant 0 = face-parallel waves, 1 = cell-parallel waves (default,
-// fastest measured at 512^3 on MI355X), 2 = streamed with a producer wave.
-std::atomic<int> g_variant {4};
-std::atomic<int> g_tile_by {0};     // 0: chosen per kernel by auto_by
-std::atomic<int> g_tile_kchunk {0}; // 0: sized per launch by auto_kchunk
+// Godunov launch configuration (process-wide): kernel variant (4 = paired rows,
+// the default), tile height (0: chosen per kernel by auto_by) and planes per block
+// (0: sized per launch). The three live in one atomic word, so every launch reads
+// a consistent triple even while another thread reconfigures; all configurations
+// give bitwise identical results, so a concurrent change affects speed only.
+struct GodunovConfig
+{
+    int variant;
+    int by;
+    int kchunk;
+};
+
+std::atomic<uint64_t> g_config {uint64_t(4) << 56};
+
+uint64_t pack_config(int variant, int by, int kchunk)
+{
+    return (uint64_t(uint8_t(variant)) << 56) | (uint64_t(uint8_t(by)) << 48) | uint64_t(uint32_t(kchunk));
+}
+
+GodunovConfig load_config()
+{
+    uint64_t const w = g_config.load();
+    return GodunovConfig {int((w >> 56) & 0xff), int((w >> 48) & 0xff), int(uint32_t(w & 0xffffffffu))};
+}
 
 bool config_supported(int variant, int by)
 {
@@ -385,10 +403,11 @@ int dispatch_tiled(
         T invdx2,
         eb::DtSink<T> const& sink)
 {
-    int const variant = g_variant.load();
-    int const by_set = g_tile_by.load();
+    GodunovConfig const cfg = load_config();
+    int const variant = cfg.variant;
+    int const by_set = cfg.by;
     int const by = by_set > 0 ? by_set : auto_by(variant, sizeof(T), Fused);
-    int const kc_set = g_tile_kchunk.load();
+    int const kc_set = cfg.kchunk;
     int const kchunk = kc_set > 0 ? kc_set
                        : variant == 4 ? static_cast<int>(pair_kchunk(((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by),
                                                                      n2 - 2, sizeof(T) == 8 ? 128 : 32))
@@ -732,9 +751,7 @@ int eb_set_godunov_config(int variant, int by, int kchunk)
                     "unsupported (variant, tile height): (0-4, 0 = automatic), (0, 2|4), (1, 4|8|16), (2, 4|8), "
                     "(3, 4|8), (4, 8|16)");
     }
-    g_variant.store(variant);
-    g_tile_by.store(by);
-    g_tile_kchunk.store(kchunk);
+    g_config.store(pack_config(variant, by, kchunk));
     return EB_OK;
 }
 
@@ -743,8 +760,8 @@ int eb_get_godunov_tile_height(int real_bytes, int fused, int* by)
     if (by == nullptr || (real_bytes != 4 && real_bytes != 8)) {
         return fail(EB_ERR_INVALID_ARGUMENT, "real_bytes must be 4 or 8 and by non-null");
     }
-    int const set = g_tile_by.load();
-    *by = set > 0 ? set : auto_by(g_variant.load(), static_cast<std::size_t>(real_bytes), fused != 0);
+    GodunovConfig const cfg = load_config();
+    *by = cfg.by > 0 ? cfg.by : auto_by(cfg.variant, static_cast<std::size_t>(real_bytes), fused != 0);
     return EB_OK;
 }
 
@@ -753,9 +770,10 @@ int eb_get_godunov_config(int* variant, int* by, int* kchunk)
     if (variant == nullptr || by == nullptr || kchunk == nullptr) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null output");
     }
-    *variant = g_variant.load();
-    *by = g_tile_by.load();
-    *kchunk = g_tile_kchunk.load();
+    GodunovConfig const cfg = load_config();
+    *variant = cfg.variant;
+    *by = cfg.by;
+    *kchunk = cfg.kchunk;
     return EB_OK;
 }
 
@@ -894,6 +912,24 @@ int abi_fill(hipStream_t s, T* a, int64_t n, T v)
     }
     hipLaunchKernelGGL((eb::k_fill<T>), dim3(pointwise_grid(n)), dim3(256), 0, s, a, n, v);
     return check_launch("fill");
+}
+
+template <class T, class PS, class CS>
+int abi_stream_probe(hipStream_t s, PS const* a, CS const* b, int64_t n)
+{
+    if (n < 0) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "negative cell count");
+    }
+    if (n == 0) {
+        return EB_OK;
+    }
+    if (prim_null(a) || cons_null(b)) {
+        return fail(EB_ERR_INVALID_ARGUMENT, "null array");
+    }
+    auto const p = prim_ptrs<T>(a);
+    eb::PrimPtrs<T const> const pc {p.d, p.p, p.ux0, p.ux1, p.ux2};
+    hipLaunchKernelGGL((eb::k_stream_probe<T>), dim3(pointwise_grid(n)), dim3(256), 0, s, pc, cons_ptrs<T>(b), n);
+    return check_launch("stream_probe");
 }
 
 template <class T, class I, class PS>
@@ -1304,6 +1340,10 @@ extern "C" {
                                T dx0, T dx1, T dx2, T offset, T d_hi, T p_hi, T d_lo, T p_lo)                        \
     {                                                                                                                \
         return abi_init_diagonal<T>(s, p, n0, n1, n2, bits, dx0, dx1, dx2, offset, d_hi, p_hi, d_lo, p_lo);         \
+    }                                                                                                                \
+    int eb_stream_probe_##SUF(hipStream_t s, const eb_prim_##SUF* a, const eb_cons_##SUF* b, int64_t n)             \
+    {                                                                                                                \
+        return abi_stream_probe<T>(s, a, b, n);                                                                      \
     }                                                                                                                \
     int eb_fill_##SUF(hipStream_t s, T* a, int64_t n, T v)                                                          \
     {                                                                                                                \

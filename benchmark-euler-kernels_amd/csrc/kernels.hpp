@@ -180,6 +180,34 @@ __global__ __launch_bounds__(256) void k_fill(T* a, int64_t n, T v)
     }
 }
 
+// Bandwidth probe, not an operator of the reference: per cell the plain flux
+// kernel's HBM traffic (five fields of `a` read, five fields of `b` read and
+// written in place, one access of sizeof(T) per field) with one add per field and
+// no tiles, halos or barriers, so its rate is what plain streaming of that traffic
+// reaches on the part.
+template <class T>
+__global__ __launch_bounds__(256) void k_stream_probe(PrimPtrs<T const> a, ConsPtrs<T> b, int64_t n)
+{
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * 256;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; idx < n; idx = grid_next(idx, n, stride)) {
+        T const x0 = a.d[idx];
+        T const x1 = a.p[idx];
+        T const x2 = a.ux0[idx];
+        T const x3 = a.ux1[idx];
+        T const x4 = a.ux2[idx];
+        T const y0 = b.d[idx];
+        T const y1 = b.e[idx];
+        T const y2 = b.mx0[idx];
+        T const y3 = b.mx1[idx];
+        T const y4 = b.mx2[idx];
+        b.d[idx] = y0 + x0;
+        b.e[idx] = y1 + x1;
+        b.mx0[idx] = y2 + x2;
+        b.mx1[idx] = y3 + x3;
+        b.mx2[idx] = y4 + x4;
+    }
+}
+
 // init_implode.hpp:16-43: sphere r^2 < 0.25^2 about the cube centre; the centre
 // coordinate is formed in double and rounded to T; ghosts included.
 template <class T, class I>

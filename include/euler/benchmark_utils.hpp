@@ -63,3 +63,12 @@ void init_from_state(
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (benchmarks/benchmark_utils.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::int_cast;
+using eb::init_from_value;
+using eb::init_from_state;
+#endif

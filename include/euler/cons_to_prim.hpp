@@ -23,3 +23,10 @@ void cons_to_prim(
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (euler_operators/cons_to_prim.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::cons_to_prim;
+#endif

@@ -376,3 +376,29 @@ void from_host(HipExecSpace const& exec_space, std::vector<T> const& src, mdspan
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (euler_operators/euler_arrays.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::EulerPrim;
+using eb::EulerCons;
+using eb::EulerFlux;
+using eb::kinetic_energy;
+using eb::internal_energy;
+using eb::to_cons;
+using eb::to_prim;
+using eb::EulerPrimArrays;
+using eb::EulerConsArrays;
+using eb::create_prim_arrays_1d;
+using eb::create_cons_arrays_1d;
+using eb::to_mdspan;
+using eb::as;
+using eb::as_const;
+using eb::data_handle;
+using eb::size;
+using eb::size_bytes;
+using eb::load;
+using eb::store;
+using eb::subspan;
+#endif

@@ -71,3 +71,10 @@ void godunov_reference_order(
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (euler_operators/godunov.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::godunov;
+#endif

@@ -78,6 +78,10 @@ EB_FUNCTION constexpr T tmax(T const a, T const b) noexcept
 
 } // namespace detail
 
+// Direction tag of the Riemann solver call (godunov.hpp:12-13).
+template <std::size_t N>
+using dir_t = std::integral_constant<std::size_t, N>;
+
 struct hllc
 {
     // Flux across a face normal to Dir between the left state qL and the right
@@ -131,3 +135,12 @@ struct hllc
 };
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (euler_operators/hllc.hpp and godunov.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::dir_t;
+using eb::get;
+using eb::hllc;
+#endif

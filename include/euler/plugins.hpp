@@ -302,3 +302,12 @@ T time_step(HipExecSpace const& exec_space,
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The overloads above under the reference's global spellings as well (a
+// using-declaration names only the overloads declared before it).
+using eb::godunov;
+using eb::cons_to_prim;
+using eb::prim_to_cons;
+using eb::time_step;
+#endif

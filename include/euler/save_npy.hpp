@@ -72,3 +72,10 @@ void save_npy(std::filesystem::path const& filename, HipExecSpace const& exec_sp
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (simulations/save_npy.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::save_npy;
+#endif

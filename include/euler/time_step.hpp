@@ -45,3 +45,10 @@ void time_step(
 }
 
 } // namespace eb
+
+#ifndef EB_NO_GLOBAL_NAMES
+// The reference declares these at global scope (euler_operators/time_step.hpp); the same
+// unqualified spellings name them here. Define EB_NO_GLOBAL_NAMES to keep them
+// in namespace eb only.
+using eb::time_step;
+#endif

@@ -76,7 +76,9 @@ const char* eb_last_error(void);
  *   `kchunk` >= 1 planes marched per block along k, or 0: chosen per launch from
  *   the grid (long chunks, shortened until the launch has ~512 blocks).
  *   Defaults (4, 0, 0).
- * All configurations give bitwise identical results. */
+ * All configurations give bitwise identical results. The triple is replaced as one
+ * atomic value: each launch reads a consistent configuration even while another
+ * thread sets a new one, and such a change can alter speed, never results. */
 int eb_set_godunov_config(int variant, int by, int kchunk);
 int eb_get_godunov_config(int* variant, int* by, int* kchunk);
 /* The tile height the current configuration launches for reals of real_bytes
@@ -215,6 +217,14 @@ int eb_init_random_f32(hipStream_t stream, const eb_prim_f32* prim, int64_t n, i
  * a[0..n) = value on the device, for one field of an array set. */
 int eb_fill_f64(hipStream_t stream, double* a, int64_t n, double value);
 int eb_fill_f32(hipStream_t stream, float* a, int64_t n, float value);
+/* ---- bandwidth probe (no counterpart in the reference) -------------------
+ * For measurement only: b.f[i] += a.f[i] for the five fields and i in [0, n),
+ * one access of the element size per field and cell, no tiles or barriers. Per
+ * cell it moves the plain godunov kernel's HBM traffic (five fields read from
+ * one array set, five read and written in place in another), so its rate is
+ * what plain streaming of that traffic reaches on the device. */
+int eb_stream_probe_f64(hipStream_t stream, const eb_prim_f64* a, const eb_cons_f64* b, int64_t n);
+int eb_stream_probe_f32(hipStream_t stream, const eb_prim_f32* a, const eb_cons_f32* b, int64_t n);
 
 
 /* ---- adjoint (no reference counterpart; the reference has no derivatives) ---

@@ -141,3 +141,96 @@ def test_argument_checks_without_gpu():
     assert b"null" in lib.eb_last_error()
     assert lib.eb_step_fused_region_f64(*common, ctypes.byref(acc), region) != EB_OK
     assert b"time_acc" in lib.eb_last_error()
+
+
+REFERENCE_SPELLED_CALLER = r"""
+#include <euler/euler.hpp>
+
+using view = eb::mdspan<double, eb::dextents<int, 3>, eb::layout_left>;
+
+int main()
+{
+    eb::DefaultExecutionSpace const exec_space;
+    PerfectGas<double> const eos(1.4);
+    UniformMesh3d<double> const mesh(0.1, 0.1, 0.1);
+    auto const prim_alloc = create_prim_arrays_1d<double>(exec_space, 1000);
+    EulerPrimArrays<view> const prim = to_mdspan<view>(prim_alloc, 10, 10, 10);
+    auto const cons_alloc = create_cons_arrays_1d<double>(exec_space, 1000);
+    EulerConsArrays<view> const cons = to_mdspan<view>(cons_alloc, 10, 10, 10);
+    init_implode(exec_space, prim, mesh);
+    prim_to_cons(exec_space, as_const(prim), cons, eos);
+    double const dt = time_step(exec_space, as_const(prim), eos, mesh);
+    godunov(exec_space, as_const(prim), cons, eos, mesh, hllc(), 0.5 * dt);
+    cons_to_prim(exec_space, as_const(cons), prim, eos);
+    EulerPrim<double> const q {1.0, 1.0, 0.0, 0.0, 0.0};
+    EulerCons<double> const u = to_cons(q, eos.internal_energy(q.d, q.p));
+    EulerPrim<double> const q2 = to_prim(u, eos.pressure(u.d, internal_energy(u)));
+    EulerFlux<double> const f = hllc()(dir_t<0>(), eos, q, q2);
+    double const g = get(dir_t<1>(), q) + kinetic_energy(q) + f.d;
+    std::size_t const b = size_bytes(prim) + size(cons);
+    auto const dh = data_handle(prim);
+    auto const sub = subspan(prim, 1, eb::full_extent, eb::full_extent);
+    EulerPrim<double> const q3 = load(prim, 1, 1, 1);
+    store(q3, prim, 2, 2, 2);
+    save_npy("x.npy", exec_space, prim.d);
+    int const n = int_cast<int>(std::size_t(10));
+    (void)g; (void)b; (void)dh; (void)sub; (void)n; (void)as<view>(prim);
+    return 0;
+}
+"""
+
+
+def test_reference_spellings_compile(tmp_path):
+    """A caller written with the reference's global names (EulerPrimArrays,
+    PerfectGas, hllc, dir_t, godunov, ... unqualified, as its drivers spell them;
+    only the Kokkos view and execution-space types are spelled eb::) compiles
+    against the headers; with EB_NO_GLOBAL_NAMES the same source does not."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "caller.cpp"
+    src.write_text(REFERENCE_SPELLED_CALLER)
+    cmd = [gxx, "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+           f"-I{ROOT / 'include'}", "-I/opt/rocm/include", str(src)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run(cmd + ["-DEB_NO_GLOBAL_NAMES"], capture_output=True, text=True)
+    assert r.returncode != 0 and "PerfectGas" in r.stderr
+
+
+def test_config_reads_are_consistent_under_concurrent_sets():
+    """eb_set_godunov_config replaces (variant, by, kchunk) as one value: readers in
+    other threads see one of the triples that were set, never a mix of two."""
+    import threading
+
+    lib = ctypes.CDLL(str(LIB))
+    triples = [(4, 8, 3), (1, 16, 77), (3, 4, 0), (0, 2, 1000)]
+    stop = threading.Event()
+    bad = []
+
+    def writer(t):
+        while not stop.is_set():
+            assert lib.eb_set_godunov_config(*t) == 0
+
+    def reader():
+        v, by, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        for _ in range(20000):
+            lib.eb_get_godunov_config(ctypes.byref(v), ctypes.byref(by), ctypes.byref(kc))
+            got = (v.value, by.value, kc.value)
+            if got not in triples and got != (4, 0, 0):
+                bad.append(got)
+
+    ws = [threading.Thread(target=writer, args=(t,)) for t in triples]
+    rs = [threading.Thread(target=reader) for _ in range(2)]
+    for t in ws + rs:
+        t.start()
+    for t in rs:
+        t.join()
+    stop.set()
+    for t in ws:
+        t.join()
+    assert lib.eb_set_godunov_config(4, 0, 0) == 0
+    assert not bad, bad[:5]

@@ -250,3 +250,31 @@ def test_index_overflow_rejected_before_launch(eb, ex):
     assert rc == _lib.EB_ERR_INVALID_ARGUMENT
     rc = _lib.lib.eb_godunov_f64(ex.handle, None, ctypes.byref(cp), 4, 4, 4, 64, 1.4, 1.0, 1.0, 1.0, 0.1)
     assert rc == _lib.EB_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_stream_probe_adds_fields(eb, ex, dtype):
+    """The bandwidth probe behind bench.py's streaming ceiling: b.f += a.f for the
+    five fields over the first n cells, the rest untouched; bad arguments refused."""
+    import ctypes
+
+    from benchmark_euler_kernels_amd import _lib
+
+    suf = "f64" if dtype == np.float64 else "f32"
+    a = random_prim((9, 7, 5), seed=31, dtype=dtype)
+    b = random_prim((9, 7, 5), seed=32, dtype=dtype)
+    A, B = to_prim(eb, a), to_cons(eb, b)
+    pa, pb = eb.operators.prim_ptrs(A), eb.operators.cons_ptrs(B)
+    n = 9 * 7 * 5 - 4
+    fn = getattr(_lib.lib, f"eb_stream_probe_{suf}")
+    assert fn(ex.handle, ctypes.byref(pa), ctypes.byref(pb), n) == _lib.EB_OK
+    got = to_host(B)
+    flat_a = a.reshape(5, -1)  # (5, n2, n1, n0) C order: memory order i + n0*(j + n1*k)
+    flat_b = b.reshape(5, -1)
+    want = flat_b.copy()
+    want[:, :n] = flat_b[:, :n] + flat_a[:, :n]
+    assert np.array_equal(got.reshape(5, -1), want)
+    assert fn(ex.handle, ctypes.byref(pa), ctypes.byref(pb), 0) == _lib.EB_OK
+    assert fn(ex.handle, ctypes.byref(pa), ctypes.byref(pb), -1) == _lib.EB_ERR_INVALID_ARGUMENT
+    assert fn(ex.handle, None, ctypes.byref(pb), n) == _lib.EB_ERR_INVALID_ARGUMENT
+    assert fn(ex.handle, ctypes.byref(pa), None, n) == _lib.EB_ERR_INVALID_ARGUMENT
