@@ -31,6 +31,13 @@ MUTANTS = [
     ("time_step_dx", "(cx2 * invdx2)", "(cx2 * dx2)"),
 ]
 
+# (name, original text, mutated text) in oracle_capi.cpp: the step loop of simulate
+LOOP_MUTANTS = [
+    ("simulate_no_cfl", "cfl * dt, n0, n1, n2);", "dt, n0, n1, n2);"),
+    ("simulate_dt_recorded_with_cfl", "dts[it] = dt;", "dts[it] = cfl * dt;"),
+    ("simulate_no_conversion", "oracle::cons_to_prim(U, P, eos, n);", ";"),
+]
+
 CHECKS = [
     lambda o: pins.test_prim_to_cons_closed_form(o),
     lambda o: pins.test_cons_to_prim_closed_form(o),
@@ -43,25 +50,30 @@ CHECKS = [
     lambda o: pins.test_godunov_telescoped_conservation(o),
     lambda o: pins.test_godunov_axis_permutation(o, (1, 0, 2)),
     lambda o: pins.test_riemann_problems_converge_to_exact(o, "sod"),
+    lambda o: pins.test_simulate_is_the_reference_loop(o),
 ]
 
 
-def build_mutant(tmp: Path, name: str, old: str, new: str) -> Path:
-    text = (SRC / "euler_oracle.hpp").read_text()
+def build_mutant(tmp: Path, name: str, old: str, new: str, fname: str = "euler_oracle.hpp") -> Path:
+    text = (SRC / fname).read_text()
     assert text.count(old) >= 1, f"mutation anchor for {name} not found"
     d = tmp / name
     d.mkdir()
-    (d / "euler_oracle.hpp").write_text(text.replace(old, new, 1))
-    shutil.copy(SRC / "oracle_capi.cpp", d / "oracle_capi.cpp")
+    for f in ("euler_oracle.hpp", "oracle_capi.cpp"):
+        if f != fname:
+            shutil.copy(SRC / f, d / f)
+    (d / fname).write_text(text.replace(old, new, 1))
     lib = d / "liboracle.so"
     subprocess.run(["g++", "-std=c++20", "-O1", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
                     "-o", str(lib), str(d / "oracle_capi.cpp")], check=True)
     return lib
 
 
-@pytest.mark.parametrize("name,old,new", MUTANTS, ids=[m[0] for m in MUTANTS])
-def test_mutant_is_caught(tmp_path, name, old, new):
-    mutant = Oracle(lib_path=build_mutant(tmp_path, name, old, new))
+@pytest.mark.parametrize("name,old,new,fname", [m + ("euler_oracle.hpp",) for m in MUTANTS]
+                         + [m + ("oracle_capi.cpp",) for m in LOOP_MUTANTS],
+                         ids=[m[0] for m in MUTANTS + LOOP_MUTANTS])
+def test_mutant_is_caught(tmp_path, name, old, new, fname):
+    mutant = Oracle(lib_path=build_mutant(tmp_path, name, old, new, fname))
     caught = []
     for i, check in enumerate(CHECKS):
         try:

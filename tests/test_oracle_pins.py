@@ -281,6 +281,36 @@ def test_godunov_telescoped_conservation(orc):
     assert np.all(np.abs(lhs - rhs) <= 1e-12 * scale + 1e-18)
 
 
+def test_simulate_is_the_reference_loop(orc):
+    """oracle.simulate against the pinned operators composed as in
+    euler_simulation.cpp:56-68: the raw dt from the current primitives (returned
+    without the CFL factor), godunov with cfl*dt, then cons_to_prim on all cells.
+    A loop that took dt after the update, applied the CFL factor twice or not at
+    all, converted before the update or touched the ghost conservatives differs."""
+    n0, n1, n2 = 9, 7, 6
+    dx = (0.11, 0.07, 0.13)
+    cfl = 0.4
+    nsteps = 4
+    prim = random_prim((n0, n1, n2), seed=8)
+    prim0 = prim.copy()
+    cons = orc.prim_to_cons(prim, G)
+    ghosts = np.ones((n2, n1, n0), dtype=bool)
+    ghosts[1:-1, 1:-1, 1:-1] = False
+    cons_ghosts0 = cons[:, ghosts].copy()
+    p_ref, c_ref = prim.copy(), cons.copy()
+    dts_ref = []
+    for _ in range(nsteps):
+        dt = orc.time_step(p_ref, G, *dx)
+        dts_ref.append(dt)
+        orc.godunov(p_ref, c_ref, G, *dx, cfl * dt)
+        orc.cons_to_prim(c_ref, G, out=p_ref)
+    dts = orc.simulate(prim, cons, G, *dx, cfl, nsteps)
+    assert np.array_equal(dts, np.array(dts_ref))
+    assert np.array_equal(prim, p_ref) and np.array_equal(cons, c_ref)
+    assert np.array_equal(cons[:, ghosts], cons_ghosts0)  # frozen ghost layer
+    assert not np.array_equal(prim[:, 1:-1, 1:-1, 1:-1], prim0[:, 1:-1, 1:-1, 1:-1])  # the interior evolves
+
+
 @pytest.mark.parametrize("perm", [(1, 0, 2), (2, 1, 0), (0, 2, 1)])
 def test_godunov_axis_permutation(orc, perm):
     """Relabelling the axes (fields and velocity components together) commutes
