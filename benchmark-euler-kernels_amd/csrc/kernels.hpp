@@ -1147,7 +1147,12 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
 // and n0-1 ride along and produce nothing). Rows wrap inside the tile, so no lane
 // is lost to a ragged x extent. Per plane k:
 //   A. each thread stages its cell (primitives, c, E); the block also stages the
-//      row below its first cell and the row above its last (2*n0 halo cells);
+//      row below its first cell and the row above its last (2*n0 halo cells).
+//      Halo cell t of each thread (t < 2*n0) is loaded one plane ahead, after
+//      the staging of the plane before, so its latency overlaps a whole plane and
+//      waiting for it never waits for the plane's own loads, which are issued
+//      later; the rare second halo cell of a thread (2*n0 > 64*NW) is loaded in
+//      place;
 //   B. each thread solves the x face and the y face on its low side (from the
 //      staged neighbours a-1 and a-n0) and the z face on its high side; the y
 //      faces above the last n0 cells and the x face after the last cell are
@@ -1221,6 +1226,15 @@ __device__ __forceinline__ T godunov_tile_lin(
         arr[4][r] = f.m2;
     };
 
+    // halo cell h: the row below the tile for h < n0, the row above it after that
+    auto halo_off = [&](int h) {
+        I const b = h < w0 ? A - n0 + h : A + NT + (h - w0);
+        return static_cast<uint32_t>(clampa(b)) * static_cast<uint32_t>(sizeof(T));
+    };
+    auto halo_slot = [&](int h) { return h < w0 ? h : w0 + NT + (h - w0); };
+    bool const has_h0 = t < 2 * w0;
+    uint32_t const h0_off = has_h0 ? halo_off(t) : 0u;
+
     Flux<T> fz_lo;
     Prim<T> q_cur = ld_prim(prim, s2 * kbeg, own);
     T c_cur = sound_speed(gamma, q_cur.d, q_cur.p);
@@ -1229,18 +1243,26 @@ __device__ __forceinline__ T godunov_tile_lin(
         Prim<T> const qm = ld_prim(prim, s2 * (kbeg - 1), own);
         fz_lo = hllc_flux<2>(qm, sound_speed(gamma, qm.d, qm.p), total_energy(qm, gm1), q_cur, c_cur, e_cur);
     }
+    Prim<T> h0 {};
+    if (has_h0) {
+        h0 = ld_prim(prim, s2 * kbeg, h0_off);
+    }
 
     for (I k = kbeg; k < kend; ++k) {
         I const plane = s2 * k;
         Prim<T> const q_next = ld_prim(prim, plane + s2, own);
         Cons<T> u = ld_cons(cons, plane, own);
 
-        // A. stage plane k: own cell, then the halo rows (2*n0 cells, h < n0 below).
+        // A. stage plane k: own cell, then the halo rows (2*n0 cells, h < n0 below);
+        // then the first halo cell of the next plane is requested (k + 1 <= n2 - 1).
         stage(w0 + t, q_cur, c_cur, e_cur);
-        for (int h = t; h < 2 * w0; h += NT) {
-            I const b = h < w0 ? A - n0 + h : A + NT + (h - w0);
-            Prim<T> const q = ld_prim(prim, plane, static_cast<uint32_t>(clampa(b)) * static_cast<uint32_t>(sizeof(T)));
-            stage(h < w0 ? h : w0 + NT + (h - w0), q, sound_speed(gamma, q.d, q.p), total_energy(q, gm1));
+        if (has_h0) {
+            stage(halo_slot(t), h0, sound_speed(gamma, h0.d, h0.p), total_energy(h0, gm1));
+            h0 = ld_prim(prim, plane + s2, h0_off);
+        }
+        for (int h = t + NT; h < 2 * w0; h += NT) {
+            Prim<T> const q = ld_prim(prim, plane, halo_off(h));
+            stage(halo_slot(h), q, sound_speed(gamma, q.d, q.p), total_energy(q, gm1));
         }
         lds_barrier();
 
