@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="profiling aid: only the batched ensemble, STEPS graph-replayed steps after autotuning")
+    ap.add_argument("--f32-only", type=int, default=0, metavar="STEPS",
+                    help="profiling aid: only the FP32 godunov step at n^3 (STEPS launches after warm-up), then "
+                         "three FP32 prim_to_cons launches on the same arrays (counter calibration)")
     ap.add_argument("--traffic-record", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
                     help="write the profiled-traffic record from two rocprofv3 --pmc counter CSVs and exit")
     ap.add_argument("--traffic-calibration", action="store_true",
@@ -543,6 +546,22 @@ def main():
         eb.init_random(ex, prim, seed=1234)
         eb.prim_to_cons(ex, prim, cons, eos)
         return prim, cons
+
+    if args.f32_only:
+        log(f"FP32 godunov only: {args.f32_only} launches at {n}^3")
+        prim32, cons32 = alloc(torch.float32)
+        dt32 = 0.5 * eb.time_step(ex, prim32, eos, mesh)
+        hllc32 = eb.hllc()
+        timer.run(lambda: eb.godunov(ex, prim32, cons32, eos, mesh, hllc32, dt32), 3)
+        ms32 = timer.run(lambda: eb.godunov(ex, prim32, cons32, eos, mesh, hllc32, dt32), args.f32_only)
+        for _ in range(3):
+            eb.prim_to_cons(ex, prim32, cons32, eos)
+        torch.cuda.synchronize()
+        bytes32 = 4 * (5 * n_arr + 10 * n_int)
+        print(json.dumps({"godunov_f32_ms": round(ms32, 4), "algorithmic_bytes": bytes32,
+                          "GBps_algorithmic": round(bytes32 / ms32 / 1e6, 1), "launches": args.f32_only}), flush=True)
+        ex.close()
+        return
 
     log(f"allocating {n}^3 FP64 state")
     prim, cons = alloc(torch.float64)
