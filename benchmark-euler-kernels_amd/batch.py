@@ -65,8 +65,8 @@ class Ensemble:
     # ------------------------------------------------------------- config ---
     def set_tiles(self, by: int, kchunk: int, layout: int = 0) -> None:
         """Tile layout 0: 64 x by columns (by 4, 8, 16); 1: runs of 64*by cells of a
-        plane (by 4, 8; every n0 <= 64*by); 2: 64 x by columns with two rows per lane
-        (by 8, 16). kchunk planes per tile, 0 = whole columns."""
+        plane (by 4, 8, and 16 in FP32; every n0 <= 64*by); 2: 64 x by columns with two
+        rows per lane (by 8, 16). kchunk planes per tile, 0 = whole columns."""
         call("eb_batch_set_tiles", self._handle, layout, by, kchunk)
         self.graph = self.per_instance_graph = None
 
