@@ -113,7 +113,8 @@ int batch_resident(int layout, int by, int* out)
 
 // Tiles of every instance for tile height `by` and `kchunk` planes per tile (0: a
 // whole column), split into queues of about equal cell count; uploads descriptors,
-// tiles and ranges (blocking; configuration time only).
+// tiles and ranges (blocking: waits for all work on the device first;
+// configuration time only).
 template <class T>
 int batch_layout(eb_batch* b, int layout, int by, int kchunk)
 {
@@ -203,6 +204,11 @@ int batch_layout(eb_batch* b, int layout, int by, int kchunk)
             sorted[t] = tiles[order[t]];
         }
         tiles.swap(sorted);
+    }
+    // Launches already enqueued, on any stream, read the descriptors replaced (and
+    // possibly reallocated) below: they finish first.
+    if (hipDeviceSynchronize() != hipSuccess) {
+        return fail(EB_ERR_HIP, "device synchronisation before the layout upload failed");
     }
     if (ntiles > b->tiles_cap) {
         if (b->tiles != nullptr) {
@@ -451,7 +457,11 @@ int batch_autotune(eb_batch* b, hipStream_t s, int parity, int reps, float* best
     }
     hipEvent_t e0 = nullptr;
     hipEvent_t e1 = nullptr;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (hipEventCreate(&e0) != hipSuccess) {
+        return fail(EB_ERR_HIP, "event creation failed");
+    }
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
         return fail(EB_ERR_HIP, "event creation failed");
     }
     int best_layout = b->layout;

@@ -334,7 +334,9 @@ int eb_batch_destroy(eb_batch* batch);
  * instance with n0 <= 64*by; layout 2: 64 x by columns with two rows of cells per lane, by in
  * {8, 16}. kchunk planes per tile (0: whole columns). Default (0, 8, 32).
  * eb_batch_autotune picks all three by timing. On failure the previous layout
- * stays in place. */
+ * stays in place. Both wait for all work on the device before replacing the
+ * tile descriptors (configuration time; not for graph capture), and a graph
+ * captured before them must be captured again. */
 int eb_batch_set_tiles(eb_batch* batch, int layout, int by, int kchunk);
 int eb_batch_get_tiles(const eb_batch* batch, int* layout, int* by, int* kchunk, int* ntiles, int* grid);
 /* Device arrays: dt = 3 reals per instance ([0], [1] ping-pong raw dt, [2] the

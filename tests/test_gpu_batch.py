@@ -109,6 +109,23 @@ def test_batch_fp32_linear_1024_cell_tiles(eb, ex, orc, index_bits, kchunk):
     ens.close()
 
 
+def test_batch_relayout_while_steps_are_in_flight(eb, ex, orc):
+    """set_tiles right after enqueued steps (no host synchronisation): the layout
+    change waits for them before it replaces the tile descriptors, so the run
+    continues bitwise equal to the oracle's loop."""
+    nsteps = 6
+    inst = _instances(eb, orc, np.float64, 32)
+    refs = _reference(orc, inst, nsteps)
+    ens, P, C = _ensemble(eb, ex, inst, 32)
+    ens.run(2)
+    ens.set_tiles(4, 1, layout=0)  # 16 times the tiles of the default layout: a larger tile list
+    ens.run(2)
+    ens.set_tiles(8, 0, layout=1)
+    ens.run(2)
+    _check(ens, P, C, refs, np.float64, nsteps, inst)
+    ens.close()
+
+
 def test_batch_per_instance_launches_and_graphs(eb, ex, orc):
     nsteps = 9
     inst = _instances(eb, orc, np.float32, 32)
