@@ -691,8 +691,10 @@ int eb_context_destroy(eb_context* ctx)
     if (ctx == nullptr) {
         return EB_OK;
     }
-    bool const ok = hipFree(ctx->scratch) == hipSuccess && hipFree(ctx->dt_slot) == hipSuccess
-                    && hipFree(ctx->partials) == hipSuccess;
+    // every buffer is released even if one release fails
+    bool ok = hipFree(ctx->scratch) == hipSuccess;
+    ok = (hipFree(ctx->dt_slot) == hipSuccess) && ok;
+    ok = (hipFree(ctx->partials) == hipSuccess) && ok;
     int const rc = ok ? EB_OK : fail(EB_ERR_HIP, "hipFree failed");
     delete ctx;
     return rc;
