@@ -598,6 +598,15 @@ def main():
     }
 
     if not args.quick:
+        # the same launches timed one by one (an event pair around each), for the
+        # median and the minimum
+        import statistics
+
+        per = [timer.run(step, 1) for _ in range(args.steps)]
+        extra["godunov_f64_per_launch_ms"] = {"median": round(statistics.median(per), 4),
+                                              "min": round(min(per), 4), "max": round(max(per), 4),
+                                              "launches": len(per)}
+
         reps = max(3, args.steps // 2)
         # Streaming probe with the flux kernel's read:write mix (two arrays read per
         # array written): c = a + b over three 1 GiB FP64 arrays, i.e. what a plain
