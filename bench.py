@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--config", type=str, default=None, help="godunov 'variant,by,kchunk'")
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="profiling aid: only the batched ensemble, STEPS graph-replayed steps after autotuning")
+    ap.add_argument("--adjoint-only", type=int, default=0, metavar="RUNS",
+                    help="profiling aid: only the adjoint workload's forward + backward, RUNS timed runs after "
+                         "two warm-up runs")
     ap.add_argument("--f32-only", type=int, default=0, metavar="STEPS",
                     help="profiling aid: only the FP32 godunov step at n^3 (STEPS launches after warm-up), then "
                          "three FP32 prim_to_cons launches on the same arrays (counter calibration)")
@@ -326,6 +329,36 @@ def batch_leg(torch, eb, ex, timer, count: int = 256, nsteps: int = 500, base_st
     }
 
 
+def adjoint_profile(torch, eb, ex, runs: int, n: int = 128, nsteps: int = 200, k: int = 20):
+    """The adjoint leg's differentiable forward and backward alone (the workload of
+    adjoint_leg, without the plain forward and the FD check), for counters."""
+    import statistics
+
+    eos = eb.PerfectGas(1.4)
+    mesh = eb.UniformMesh3d(1.0 / n, 1.0 / n, 1.0 / n)
+    P0 = eb.workloads.adjoint_state(ex, n)
+    c = (torch.arange(n + 2, dtype=torch.float64, device=P0.d.device) - 0.5) / n
+    inner = (c > 0) & (c < 0.25)
+    mask = inner[:, None, None] & inner[None, :, None] & inner[None, None, :]
+
+    def fwd_bwd():
+        ts = [t.clone().requires_grad_(True) for t in P0.tensors()]
+        out = eb.adjoint.rollout(eb.EulerPrimArrays(*ts, index_bits=P0.index_bits), eos, mesh, 0.5, nsteps, k,
+                                 exec_space=ex)
+        (out[0] * out[0] * mask).sum().backward()
+
+    for _ in range(2):
+        fwd_bwd()
+    ms = []
+    for _ in range(runs):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fwd_bwd()
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return {"forward_plus_adjoint_ms_median": round(statistics.median(ms), 2), "runs": runs}
+
+
 def batch_profile(torch, eb, ex, timer, nsteps: int, count: int = 256):
     """The measured ensemble stepped nsteps times after autotuning (for counters)."""
     prims, conss, eoss, meshes, cfls, params = eb.batch.random_ensemble(ex, count, seed=1234, dtype=torch.float32,
@@ -527,6 +560,11 @@ def main():
     torch.cuda.set_device(dev)
     ex = eb.ExecSpace(dev)
     timer = Timer(torch, ex.stream)
+    if args.adjoint_only:
+        log(f"adjoint workload only: {args.adjoint_only} forward + backward runs")
+        print(json.dumps(adjoint_profile(torch, eb, ex, args.adjoint_only)), flush=True)
+        ex.close()
+        return
     if args.batch_only:
         log(f"batched ensemble only: {args.batch_only} graph-replayed steps")
         print(json.dumps(batch_profile(torch, eb, ex, timer, args.batch_only)), flush=True)
