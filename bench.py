@@ -718,7 +718,17 @@ def main():
             "ms": round(ms_sp32, 4), "GBps": round(sp32_gbs, 1), "hbm_frac": round(sp32_gbs / HBM_PEAK_GBS, 4),
             "godunov_f32_frac_of_probe": round(g32 / sp32_gbs, 4),
             "probe_speedup_f32_vs_f64": round(ms_sprobe / ms_sp32, 3)}
-        del prim32, cons32
+        # Fused full step in FP32 on the same state (graph-replayed pairs).
+        sim32 = eb.simulation.Simulation(ex, prim32, cons32, eos, mesh, 0.5)
+        sim32.run(4, use_graph=True)
+        ms_fused32 = timer.run(lambda: sim32.run(2, use_graph=True), max(2, reps // 2)) / 2
+        extra["step_fused_f32"] = {
+            "ms": round(ms_fused32, 4),
+            "Mcell_updates_per_s": round(n_int / ms_fused32 / 1e3, 1),
+            "GBps_algorithmic": round(4 * (5 * n_arr + 15 * n_int) / ms_fused32 / 1e6, 1),
+            "speedup_vs_f64": round(ms_fused / ms_fused32, 3),
+        }
+        del sim32, prim32, cons32
         torch.cuda.empty_cache()
 
     if not args.quick and not args.no_fp32_report:

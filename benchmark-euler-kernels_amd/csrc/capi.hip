@@ -320,7 +320,15 @@ void launch_tiled(
 // get one round of short chunks (64^3: 1-plane chunks, 256 blocks, 12 us against
 // 45 us with 8-plane chunks and 32 blocks); long chunks save the plane a block
 // re-reads below its chunk. Measured lmax: 128 planes in FP64 (128^3: 8 planes;
-// 320^3: 64; 512^3: 128), 32 in FP32 (faster at 320^3, level at 256^3 and 512^3).
+// 320^3: 64; 512^3: 128), 32 in the FP32 flux kernel (faster at 400^3 and 448^3,
+// level at 256^3, 320^3 and 512^3), 128 in the fused step in both precisions (every
+// block commits to the one time-step word, so fewer blocks pay: FP32 at 512^3
+// 2.70 against 2.91 ms with 32-plane chunks, 256^3 0.353 against 0.383 ms).
+int64_t pair_lmax(std::size_t real_bytes, bool fused)
+{
+    return real_bytes == 8 || fused ? 128 : 32;
+}
+
 int64_t pair_kchunk(int64_t columns, int64_t nz, int64_t lmax, int64_t slots = 256)
 {
     if (nz < 1) {
@@ -351,7 +359,7 @@ int launch_fused_region(hipStream_t s, eb::PrimPtrs<T const> const& pc, eb::Cons
     int64_t const y0 = (lo[1] - 1) / BY;
     int64_t const y1 = (hi[1] - 1) / BY + 1;
     int64_t const columns = (x1 - x0) * (y1 - y0);
-    int const kchunk = static_cast<int>(pair_kchunk(columns, hi[2] - lo[2] + 1, sizeof(T) == 8 ? 128 : 32));
+    int const kchunk = static_cast<int>(pair_kchunk(columns, hi[2] - lo[2] + 1, pair_lmax(sizeof(T), true)));
     int64_t const z0 = (lo[2] - 1) / kchunk;
     int64_t const z1 = (hi[2] - 1) / kchunk + 1;
     int64_t const nblk = columns * (z1 - z0);
@@ -410,7 +418,7 @@ int dispatch_tiled(
     int const kc_set = cfg.kchunk;
     int const kchunk = kc_set > 0 ? kc_set
                        : variant == 4 ? static_cast<int>(pair_kchunk(((n0 - 2 + 63) / 64) * ((n1 - 2 + by - 1) / by),
-                                                                     n2 - 2, sizeof(T) == 8 ? 128 : 32))
+                                                                     n2 - 2, pair_lmax(sizeof(T), Fused)))
                                       : auto_kchunk(n0, n1, n2, by);
     // The block count must fit a 32-bit grid and a plane must fit 32-bit offsets.
     int64_t const columns = variant == 3 ? (n0 * (n1 - 2) + 64 * by - 1) / (64 * by)

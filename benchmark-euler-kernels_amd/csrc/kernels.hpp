@@ -369,13 +369,15 @@ __global__ __launch_bounds__(256) void k_godunov_naive(
 
 // Block-to-tile map that keeps neighbouring tiles on one XCD: the hardware deals
 // workgroups round-robin over the 8 XCDs, so block b runs on XCD b%8; giving each
-// XCD a contiguous range of tiles lets halo re-reads hit that XCD's L2.
+// XCD a contiguous range of tiles lets halo re-reads hit that XCD's L2. With
+// nblk = 8q + r, XCD x takes the q + (x < r) tiles from x*q + min(x, r) on, in block
+// order, so the map is a bijection of [0, nblk) for every block count.
 __device__ __forceinline__ int xcd_tile(int b, int nblk)
 {
-    if ((nblk & 7) != 0) {
-        return b;
-    }
-    return (b & 7) * (nblk >> 3) + (b >> 3);
+    int const q = nblk >> 3;
+    int const r = nblk & 7;
+    int const x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Hides how a wave-uniform value was computed from the optimiser (an empty asm

@@ -145,6 +145,35 @@ def test_godunov_tiled(eb, ex, orc, dtype, index_bits, shape, tile):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(130, 70, 100), (190, 50, 77)])
+def test_godunov_block_counts_off_the_xcd_multiple(eb, ex, orc, dtype, shape):
+    """Default configuration where the launch has 8q + r blocks with r != 0 (250
+    and 225 blocks of 64 x 16 tiles): the block-to-tile map over the eight XCDs
+    must still cover every tile once. One godunov step, then two fused steps."""
+    print(f"godunov + fused steps default config {dtype.__name__} shape={shape}", flush=True)
+    gamma, cfl = 1.4, 0.5
+    n0, n1, n2 = shape
+    dx = (1.0 / (n0 - 2), 1.0 / (n1 - 2), 1.0 / (n2 - 2))
+    prim = random_prim(shape, seed=25, dtype=dtype)
+    cons = orc.prim_to_cons(prim, gamma)
+    dt = 0.5 * orc.time_step(prim, gamma, *dx)
+    ref = cons.copy()
+    orc.godunov(prim, ref, gamma, *dx, dt)
+    P, C = to_prim(eb, prim), to_cons(eb, cons)
+    eb.godunov(ex, P, C, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx), eb.hllc(), float(dt))
+    assert_parity(to_host(C), ref, dtype, "godunov")
+
+    p_ref, c_ref = prim.copy(), cons.copy()
+    orc.simulate(p_ref, c_ref, gamma, *dx, cfl, 2)
+    P, C = to_prim(eb, prim, 64), to_cons(eb, cons, 64)
+    sim = eb.simulation.Simulation(ex, P, C, eb.PerfectGas(gamma), eb.UniformMesh3d(*dx), cfl)
+    sim.run(2)
+    sim.finish()
+    assert_parity(to_host(C), c_ref, dtype, "fused steps (cons)")
+    assert_parity(to_host(P), p_ref, dtype, "fused steps (prim)")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_godunov_reference_order(eb, ex, orc, dtype):
     gamma = 1.4
     shape = (70, 19, 13)
