@@ -145,152 +145,157 @@ __device__ __forceinline__ void hllc_vjp(
     T const eo = ratio * etot + X;
     T const mass = d_o * uo;
 
-    // ---- reverse: outputs F = (mass, (eo+po)*uo, mass*u_t, mass*uo + po)
-    T const g_mn = flux_normal<Dir>(g);
-    T g_mass = g.d + g_mn * uo;
-    T g_uo = g.e * (eo + po) + g_mn * mass;
-    T g_po = g.e * uo + g_mn;
-    T const g_eo = g.e * uo;
-    // transverse momentum fluxes mass * q.u_t
-    PrimGrad<T> gq {T(0), T(0), T(0), T(0), T(0)};
-    if constexpr (Dir != 0) {
-        g_mass += g.m0 * q.u0;
-        gq.u0 += g.m0 * mass;
-    }
-    if constexpr (Dir != 1) {
-        g_mass += g.m1 * q.u1;
-        gq.u1 += g.m1 * mass;
-    }
-    if constexpr (Dir != 2) {
-        g_mass += g.m2 * q.u2;
-        gq.u2 += g.m2 * mass;
-    }
-    // mass = d_o * uo
-    T const g_do = g_mass * uo;
-    g_uo += g_mass * d_o;
-    // eo = ratio * etot + X
-    T g_ratio = g_eo * etot;
-    T const g_etot = g_eo * ratio;
-    // X = C / D
-    T const g_C = g_eo * rD;
-    T const g_D = -g_C * X;
-    // C = po*uo - q.p*un
-    g_po += g_C * uo;
-    g_uo += g_C * po;
-    gq.p += -g_C * un;
-    T g_un = -g_C * q.p;
-    // D = S - us
-    T g_S = g_D;
-    T g_us = -g_D;
-    // d_o = ratio * q.d
-    g_ratio += g_do * q.d;
-    gq.d += g_do * ratio;
-    // ratio = A / B; A = S - un; B = S - uo
-    T const g_A = g_ratio * rB;
-    T const g_B = -g_A * ratio;
-    g_S += g_A + g_B;
-    g_un += -g_A;
-    g_uo += -g_B;
-    // uo, po selections
-    T g_ps = T(0);
-    if (sup) {
-        g_un += g_uo;
-        gq.p += g_po;
-    } else {
-        g_us += g_uo;
-        g_ps = g_po;
-    }
-    // etot = p/gm1 + d*|u|^2/2 of the upwind state
+    // Reverse sweep. Its products may contract into FMAs (they need not round like
+    // the forward step); the branch decisions above are taken without contraction.
     {
-        T const u2 = (q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2);
-        gq.p += g_etot * igm1;
-        gq.d += g_etot * u2 * T(0.5);
-        gq.u0 += g_etot * q.d * q.u0;
-        gq.u1 += g_etot * q.d * q.u1;
-        gq.u2 += g_etot * q.d * q.u2;
-    }
-    comp<Dir>(gq) += g_un;
-    // route the upwind state's and S's adjoints to the chosen side
-    T g_SL = T(0);
-    T g_SR = T(0);
-    if (left) {
-        g_SL += g_S;
-        gl.d += gq.d;
-        gl.p += gq.p;
-        gl.u0 += gq.u0;
-        gl.u1 += gq.u1;
-        gl.u2 += gq.u2;
-    } else {
-        g_SR += g_S;
-        gr.d += gq.d;
-        gr.p += gq.p;
-        gr.u0 += gq.u0;
-        gr.u1 += gq.u1;
-        gr.u2 += gq.u2;
-    }
-    // ps = 0.5*((pL + pR) + rcL*bL + rcR*bR)
-    T const h = T(0.5) * g_ps;
-    T g_pL = h;
-    T g_pR = h;
-    T g_rcL = h * bL;
-    T g_rcR = h * bR;
-    T const g_bL = h * rcL;
-    T const g_bR = h * rcR;
-    g_us += g_bL + g_bR;
-    T g_unL = -g_bL;
-    T g_unR = -g_bR;
-    // us = num / den
-    T const g_num = g_us * rden;
-    T const g_den = -g_num * us;
-    // num = (pR - pL) + rcL*unL - rcR*unR; den = rcL - rcR
-    g_pR += g_num;
-    g_pL -= g_num;
-    g_rcL += g_num * unL + g_den;
-    g_unL += g_num * rcL;
-    g_rcR += -g_num * unR - g_den;
-    g_unR += -g_num * rcR;
-    // rcL = dL*aL; rcR = dR*aR
-    T g_dL = g_rcL * aL;
-    T g_dR = g_rcR * aR;
-    T const g_aL = g_rcL * qL.d;
-    T const g_aR = g_rcR * qR.d;
-    // aL = SL - unL; aR = SR - unR
-    g_SL += g_aL;
-    g_unL -= g_aL;
-    g_SR += g_aR;
-    g_unR -= g_aR;
-    // SL = umin - cmax; SR = umax + cmax
-    T const g_cmax = g_SR - g_SL;
-    if (sel_min) {
-        g_unR += g_SL;
-    } else {
-        g_unL += g_SL;
-    }
-    if (sel_max) {
-        g_unR += g_SR;
-    } else {
-        g_unL += g_SR;
-    }
-    // c = sqrt(gamma*p/d): dc = c/2 * (dp/p - dd/d), on the side that set cmax
-    {
-        Prim<T> const& qc = sel_c ? qR : qL;
-        T const gcc = g_cmax * cmax;
-        T const gp = gcc * (T(0.5) * fast_rcp(qc.p));
-        T const gd = gcc * (T(0.5) * fast_rcp(qc.d));
-        if (sel_c) {
-            g_pR += gp;
-            g_dR -= gd;
-        } else {
-            g_pL += gp;
-            g_dL -= gd;
+#pragma clang fp contract(fast)
+        // ---- reverse: outputs F = (mass, (eo+po)*uo, mass*u_t, mass*uo + po)
+        T const g_mn = flux_normal<Dir>(g);
+        T g_mass = g.d + g_mn * uo;
+        T g_uo = g.e * (eo + po) + g_mn * mass;
+        T g_po = g.e * uo + g_mn;
+        T const g_eo = g.e * uo;
+        // transverse momentum fluxes mass * q.u_t
+        PrimGrad<T> gq {T(0), T(0), T(0), T(0), T(0)};
+        if constexpr (Dir != 0) {
+            g_mass += g.m0 * q.u0;
+            gq.u0 += g.m0 * mass;
         }
+        if constexpr (Dir != 1) {
+            g_mass += g.m1 * q.u1;
+            gq.u1 += g.m1 * mass;
+        }
+        if constexpr (Dir != 2) {
+            g_mass += g.m2 * q.u2;
+            gq.u2 += g.m2 * mass;
+        }
+        // mass = d_o * uo
+        T const g_do = g_mass * uo;
+        g_uo += g_mass * d_o;
+        // eo = ratio * etot + X
+        T g_ratio = g_eo * etot;
+        T const g_etot = g_eo * ratio;
+        // X = C / D
+        T const g_C = g_eo * rD;
+        T const g_D = -g_C * X;
+        // C = po*uo - q.p*un
+        g_po += g_C * uo;
+        g_uo += g_C * po;
+        gq.p += -g_C * un;
+        T g_un = -g_C * q.p;
+        // D = S - us
+        T g_S = g_D;
+        T g_us = -g_D;
+        // d_o = ratio * q.d
+        g_ratio += g_do * q.d;
+        gq.d += g_do * ratio;
+        // ratio = A / B; A = S - un; B = S - uo
+        T const g_A = g_ratio * rB;
+        T const g_B = -g_A * ratio;
+        g_S += g_A + g_B;
+        g_un += -g_A;
+        g_uo += -g_B;
+        // uo, po selections
+        T g_ps = T(0);
+        if (sup) {
+            g_un += g_uo;
+            gq.p += g_po;
+        } else {
+            g_us += g_uo;
+            g_ps = g_po;
+        }
+        // etot = p/gm1 + d*|u|^2/2 of the upwind state
+        {
+            T const u2 = (q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2);
+            gq.p += g_etot * igm1;
+            gq.d += g_etot * u2 * T(0.5);
+            gq.u0 += g_etot * q.d * q.u0;
+            gq.u1 += g_etot * q.d * q.u1;
+            gq.u2 += g_etot * q.d * q.u2;
+        }
+        comp<Dir>(gq) += g_un;
+        // route the upwind state's and S's adjoints to the chosen side
+        T g_SL = T(0);
+        T g_SR = T(0);
+        if (left) {
+            g_SL += g_S;
+            gl.d += gq.d;
+            gl.p += gq.p;
+            gl.u0 += gq.u0;
+            gl.u1 += gq.u1;
+            gl.u2 += gq.u2;
+        } else {
+            g_SR += g_S;
+            gr.d += gq.d;
+            gr.p += gq.p;
+            gr.u0 += gq.u0;
+            gr.u1 += gq.u1;
+            gr.u2 += gq.u2;
+        }
+        // ps = 0.5*((pL + pR) + rcL*bL + rcR*bR)
+        T const h = T(0.5) * g_ps;
+        T g_pL = h;
+        T g_pR = h;
+        T g_rcL = h * bL;
+        T g_rcR = h * bR;
+        T const g_bL = h * rcL;
+        T const g_bR = h * rcR;
+        g_us += g_bL + g_bR;
+        T g_unL = -g_bL;
+        T g_unR = -g_bR;
+        // us = num / den
+        T const g_num = g_us * rden;
+        T const g_den = -g_num * us;
+        // num = (pR - pL) + rcL*unL - rcR*unR; den = rcL - rcR
+        g_pR += g_num;
+        g_pL -= g_num;
+        g_rcL += g_num * unL + g_den;
+        g_unL += g_num * rcL;
+        g_rcR += -g_num * unR - g_den;
+        g_unR += -g_num * rcR;
+        // rcL = dL*aL; rcR = dR*aR
+        T g_dL = g_rcL * aL;
+        T g_dR = g_rcR * aR;
+        T const g_aL = g_rcL * qL.d;
+        T const g_aR = g_rcR * qR.d;
+        // aL = SL - unL; aR = SR - unR
+        g_SL += g_aL;
+        g_unL -= g_aL;
+        g_SR += g_aR;
+        g_unR -= g_aR;
+        // SL = umin - cmax; SR = umax + cmax
+        T const g_cmax = g_SR - g_SL;
+        if (sel_min) {
+            g_unR += g_SL;
+        } else {
+            g_unL += g_SL;
+        }
+        if (sel_max) {
+            g_unR += g_SR;
+        } else {
+            g_unL += g_SR;
+        }
+        // c = sqrt(gamma*p/d): dc = c/2 * (dp/p - dd/d), on the side that set cmax
+        {
+            Prim<T> const& qc = sel_c ? qR : qL;
+            T const gcc = g_cmax * cmax;
+            T const gp = gcc * (T(0.5) * fast_rcp(qc.p));
+            T const gd = gcc * (T(0.5) * fast_rcp(qc.d));
+            if (sel_c) {
+                g_pR += gp;
+                g_dR -= gd;
+            } else {
+                g_pL += gp;
+                g_dL -= gd;
+            }
+        }
+        gl.d += g_dL;
+        gl.p += g_pL;
+        comp<Dir>(gl) += g_unL;
+        gr.d += g_dR;
+        gr.p += g_pR;
+        comp<Dir>(gr) += g_unR;
     }
-    gl.d += g_dL;
-    gl.p += g_pL;
-    comp<Dir>(gl) += g_unL;
-    gr.d += g_dR;
-    gr.p += g_pR;
-    comp<Dir>(gr) += g_unR;
 }
 
 // Same, from the primitives alone.
@@ -311,6 +316,7 @@ __device__ __forceinline__ void hllc_vjp(
 template <class T>
 __device__ __forceinline__ Cons<T> c2p_vjp(Prim<T> const& q, PrimGrad<T> const& g, T gm1)
 {
+#pragma clang fp contract(fast)
     T const inv = fast_rcp(q.d);
     T const ke = q.d * ((q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2)) * T(0.5);
     T const g_ke = -gm1 * g.p;
@@ -327,6 +333,7 @@ __device__ __forceinline__ Cons<T> c2p_vjp(Prim<T> const& q, PrimGrad<T> const& 
 template <class T>
 __device__ __forceinline__ PrimGrad<T> p2c_vjp(Prim<T> const& q, Cons<T> const& g, T gm1)
 {
+#pragma clang fp contract(fast)
     T const u2 = (q.u0 * q.u0) + (q.u1 * q.u1) + (q.u2 * q.u2);
     PrimGrad<T> r;
     r.d = g.d + g.e * u2 * T(0.5) + g.m0 * q.u0 + g.m1 * q.u1 + g.m2 * q.u2;

@@ -74,6 +74,7 @@ template <class T, class I, int Mode>
 __device__ __forceinline__ void vjp_output(ConsPtrs<T> const& mu_out, PrimPtrs<T> const& gprim, I c, Prim<T> const& q,
                                            Flux<T> const& w, PrimGrad<T> const& gq, T dtodv, T gm1)
 {
+#pragma clang fp contract(fast)
     PrimGrad<T> const gp {-dtodv * gq.d, -dtodv * gq.p, -dtodv * gq.u0, -dtodv * gq.u1, -dtodv * gq.u2};
     if constexpr (Mode == 0) {
         Cons<T> const r = c2p_vjp(q, gp, gm1);
