@@ -393,6 +393,32 @@ __device__ __forceinline__ void uniform_opaque(I& v)
 #endif
 }
 
+// The same for a per-lane 32-bit byte offset of a march, redefined at each use
+// site (no instruction): the offset's zero extension then stays next to the
+// accesses, where instruction selection folds it into the saddr + voffset form of
+// each global load and store, instead of a loop-invariant 64-bit copy of the
+// offset added to every field's plane base by a 64-bit VALU add per access (and
+// 64-bit addresses held in VGPRs from a load to the store that reuses them).
+__device__ __forceinline__ void lane_opaque(uint32_t& v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(v));
+#endif
+}
+
+// Pins the five values of a conservative update where they are formed. Without
+// it the compiler may sink the update, and with it the wait for the conservative
+// loads, into the store's validity branch; a wave that skips the branch then
+// reaches the next plane with those loads still counted in flight, and every
+// plane starts with a wait that also covers the previous plane's stores.
+template <class T>
+__device__ __forceinline__ void keep_here(Cons<T>& u)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(u.d), "+v"(u.e), "+v"(u.m0), "+v"(u.m1), "+v"(u.m2));
+#endif
+}
+
 // Pointer into the global address space. Every field array is device memory, so
 // accesses through it are global loads and stores even where the pointer was read
 // from memory (the batched ensemble's instance descriptors), where the compiler
@@ -855,8 +881,13 @@ __device__ __forceinline__ T godunov_tile_pair(
     I const ic = (i < n0 - 1) ? i : n0 - 1;
     I const jAc = (jA < n1 - 1) ? jA : n1 - 1;
     I const jBc = (jB < n1 - 1) ? jB : n1 - 1;
-    uint32_t const offA = static_cast<uint32_t>(ic + n0 * jAc) * static_cast<uint32_t>(sizeof(T));
-    uint32_t const offB = static_cast<uint32_t>(ic + n0 * jBc) * static_cast<uint32_t>(sizeof(T));
+    uint32_t offA = static_cast<uint32_t>(ic + n0 * jAc) * static_cast<uint32_t>(sizeof(T));
+    uint32_t offB = static_cast<uint32_t>(ic + n0 * jBc) * static_cast<uint32_t>(sizeof(T));
+    // Plane-relative addressing pinned at every access site (lane_opaque), in the
+    // flux kernel: 4% (FP64) and 13% (FP32) fewer VGPRs, 1.5% faster at 512^3.
+    // The fused step keeps the compiler's own addressing: with fewer VGPRs its FP32
+    // form would fit two 8-wave blocks per CU, which this march runs 10% slower.
+    constexpr bool pin_offsets = !Fused;
 
     int hrow = 0;
     int hcol = 0;
@@ -928,10 +959,20 @@ __device__ __forceinline__ T godunov_tile_pair(
     if (has_halo) {
         h_cur = ld_prim(prim, s2 * kbeg, halo);
     }
+    if constexpr (pin_offsets) {
+        // the prologue's loads land before the march, so that no register is
+        // still pending at the loop head (a wait there would run at every plane)
+        wait_vmem_all();
+    }
 
     for (I k = kbeg; k < kend; ++k) {
         I plane = s2 * k;
         uniform_opaque(plane);
+        if constexpr (pin_offsets) {
+            lane_opaque(offA);
+            lane_opaque(offB);
+            lane_opaque(halo);
+        }
         Prim<T> const qA_next = ld_prim(prim, plane + s2, offA);
         Prim<T> const qB_next = ld_prim(prim, plane + s2, offB);
         Prim<T> h_next {};
@@ -994,6 +1035,10 @@ __device__ __forceinline__ T godunov_tile_pair(
         Flux<T> const fxB_lo = hllc_flux<0>(cell(rB, tx), sq[5][rB][tx], sq[6][rB][tx], cell(rB, tx + 1),
                                             sq[5][rB][tx + 1], sq[6][rB][tx + 1]);
         Flux<T> fxB_hi = shfl_flux(fxB_lo);
+        if constexpr (pin_offsets) {
+            lane_opaque(offA);
+            lane_opaque(offB);
+        }
         Cons<T> uA = ld_cons(cons, plane, offA);
         Cons<T> uB = ld_cons(cons, plane, offB);
         lds_barrier();
@@ -1017,6 +1062,12 @@ __device__ __forceinline__ T godunov_tile_pair(
         if (apply) {
             apply_update(uA, dtodv, accA);
             apply_update(uB, dtodv, accB);
+        }
+        if constexpr (pin_offsets) {
+            keep_here(uA);
+            keep_here(uB);
+            lane_opaque(offA);
+            lane_opaque(offB);
         }
         if (validA) {
             st_cons(cons, plane, offA, uA);
