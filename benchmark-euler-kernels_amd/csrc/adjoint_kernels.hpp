@@ -69,28 +69,67 @@ __device__ __forceinline__ void add_to(PrimGrad<T>& a, PrimGrad<T> const& b)
     a.u2 += b.u2;
 }
 
-// Output of one cell from its gathered face adjoints gq (see the header comment).
-template <class T, class I, int Mode>
-__device__ __forceinline__ void vjp_output(ConsPtrs<T> const& mu_out, PrimPtrs<T> const& gprim, I c, Prim<T> const& q,
-                                           Flux<T> const& w, PrimGrad<T> const& gq, T dtodv, T gm1)
+// Output of one cell from its gathered face adjoints gq (see the header comment):
+// the five values, in the order of the destination's fields (mu_out's d, e, mx0,
+// mx1, mx2 in mode 0, gprim's d, p, ux0, ux1, ux2 in mode 1), handed to `put`.
+template <class T, int Mode, class Put>
+__device__ __forceinline__ void vjp_emit(Prim<T> const& q, Flux<T> const& w, PrimGrad<T> const& gq, T dtodv, T gm1,
+                                         Put&& put)
 {
 #pragma clang fp contract(fast)
     PrimGrad<T> const gp {-dtodv * gq.d, -dtodv * gq.p, -dtodv * gq.u0, -dtodv * gq.u1, -dtodv * gq.u2};
     if constexpr (Mode == 0) {
         Cons<T> const r = c2p_vjp(q, gp, gm1);
-        mu_out.d[c] = w.d + r.d;
-        mu_out.e[c] = w.e + r.e;
-        mu_out.mx0[c] = w.m0 + r.m0;
-        mu_out.mx1[c] = w.m1 + r.m1;
-        mu_out.mx2[c] = w.m2 + r.m2;
+        put(w.d + r.d, w.e + r.e, w.m0 + r.m0, w.m1 + r.m1, w.m2 + r.m2);
     } else {
         PrimGrad<T> const r = p2c_vjp(q, Cons<T> {w.d, w.e, w.m0, w.m1, w.m2}, gm1);
-        gprim.d[c] = gp.d + r.d;
-        gprim.p[c] = gp.p + r.p;
-        gprim.ux0[c] = gp.u0 + r.u0;
-        gprim.ux1[c] = gp.u1 + r.u1;
-        gprim.ux2[c] = gp.u2 + r.u2;
+        put(gp.d + r.d, gp.p + r.p, gp.u0 + r.u0, gp.u1 + r.u1, gp.u2 + r.u2);
     }
+}
+
+// The same output stored at linear cell index c.
+template <class T, class I, int Mode>
+__device__ __forceinline__ void vjp_output(ConsPtrs<T> const& mu_out, PrimPtrs<T> const& gprim, I c, Prim<T> const& q,
+                                           Flux<T> const& w, PrimGrad<T> const& gq, T dtodv, T gm1)
+{
+    vjp_emit<T, Mode>(q, w, gq, dtodv, gm1, [&](T v0, T v1, T v2, T v3, T v4) {
+        if constexpr (Mode == 0) {
+            mu_out.d[c] = v0;
+            mu_out.e[c] = v1;
+            mu_out.mx0[c] = v2;
+            mu_out.mx1[c] = v3;
+            mu_out.mx2[c] = v4;
+        } else {
+            gprim.d[c] = v0;
+            gprim.p[c] = v1;
+            gprim.ux0[c] = v2;
+            gprim.ux1[c] = v3;
+            gprim.ux2[c] = v4;
+        }
+    });
+}
+
+// The same output stored plane-relative: scalar plane base, 32-bit lane byte offset.
+template <class T, class I, int Mode>
+__device__ __forceinline__ void vjp_output_at(ConsPtrs<T> const& mu_out, PrimPtrs<T> const& gprim, I plane,
+                                              uint32_t off, Prim<T> const& q, Flux<T> const& w,
+                                              PrimGrad<T> const& gq, T dtodv, T gm1)
+{
+    vjp_emit<T, Mode>(q, w, gq, dtodv, gm1, [&](T v0, T v1, T v2, T v3, T v4) {
+        if constexpr (Mode == 0) {
+            st(mu_out.d + plane, off, v0);
+            st(mu_out.e + plane, off, v1);
+            st(mu_out.mx0 + plane, off, v2);
+            st(mu_out.mx1 + plane, off, v3);
+            st(mu_out.mx2 + plane, off, v4);
+        } else {
+            st(gprim.d + plane, off, v0);
+            st(gprim.p + plane, off, v1);
+            st(gprim.ux0 + plane, off, v2);
+            st(gprim.ux1 + plane, off, v3);
+            st(gprim.ux2 + plane, off, v4);
+        }
+    });
 }
 
 // Ghost cells: each ghost cell on a face of the array (not an edge or corner)
@@ -292,7 +331,7 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
     bool const own_int = col_int(i) && row_ok;
     I const ic = (i < n0 - 1) ? i : n0 - 1;
     I const jc = (j < n1 - 1) ? j : n1 - 1;
-    uint32_t const own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
+    uint32_t own = static_cast<uint32_t>(ic + n0 * jc) * static_cast<uint32_t>(sizeof(T));
 
     int hrow = 0;
     int hcol = 0;
@@ -355,16 +394,41 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         PrimGrad<T> dummy = zero;
         hllc_vjp<2>(vm, v_cur, gm1, face_weight(ds2, wm, own_int && kbeg - 1 >= 1, w_cur, own_int), dummy, carry);
     }
+    // Loads run one plane ahead of their use: the own cell of plane k+1 and the halo
+    // of plane k are in registers when plane k starts, so the test of whether the
+    // plane is active (which needs the adjoint of plane k+1 and of the halo) does not
+    // wait for memory. Planes past the chunk are never loaded: the prefetch plane is
+    // clamped to kend (<= n2-1), whose values are then not used.
+    Prim<T> q_nx {};
+    Flux<T> w_nx {};
+    Prim<T> h_q {};
+    Flux<T> h_w {};
+    if (kbeg < kend) {
+        q_nx = ld_prim(prim, s2 * (kbeg + 1), own);
+        w_nx = load_mu(s2 * (kbeg + 1), own);
+        if (has_halo) {
+            h_q = ld_prim(prim, s2 * kbeg, halo);
+            h_w = load_mu(s2 * kbeg, halo);
+        }
+    }
 
     for (I k = kbeg; k < kend; ++k) {
-        I const plane = s2 * k;
-        Prim<T> const q_next = ld_prim(prim, plane + s2, own);
-        Flux<T> const w_next = load_mu(plane + s2, own);
-        Prim<T> h_q {};
-        Flux<T> h_w {};
+        I plane = s2 * k;
+        uniform_opaque(plane);
+        lane_opaque(own);
+        lane_opaque(halo);
+        Prim<T> const q_next = q_nx;
+        Flux<T> const w_next = w_nx;
+        {
+            I const kp = k + 2 <= kend ? k + 2 : kend;
+            q_nx = ld_prim(prim, s2 * kp, own);
+            w_nx = load_mu(s2 * kp, own);
+        }
+        Prim<T> h_qn {};
+        Flux<T> h_wn {};
         if (has_halo) {
-            h_q = ld_prim(prim, plane, halo);
-            h_w = load_mu(plane, halo);
+            h_qn = ld_prim(prim, plane + s2, halo);
+            h_wn = load_mu(plane + s2, halo);
         }
 
         // A. stage plane k. Where the incoming adjoint and the carried z-face part
@@ -379,12 +443,14 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         }
         if (__syncthreads_or(active ? 1 : 0) == 0) {
             if (own_int) {
-                vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), cell(ty + 1, tx + 1).q,
-                                       weight(ty + 1, tx + 1), carry, dtodv, gm1);
+                vjp_output_at<T, I, Mode>(mu_out, gprim, plane, own, cell(ty + 1, tx + 1).q, weight(ty + 1, tx + 1),
+                                          carry, dtodv, gm1);
             }
             carry = zero;
             v_cur = vjp_cell(q_next, gamma);
             w_cur = w_next;
+            h_q = h_qn;
+            h_w = h_wn;
             continue;
         }
 
@@ -463,14 +529,17 @@ __global__ __launch_bounds__(64 * BY, EB_VJP_MIN_WAVES) void k_godunov_vjp_tiled
         add_to(gq, hx);
         add_to(gq, PrimGrad<T> {sgy[0][ty + 1][tx], sgy[1][ty + 1][tx], sgy[2][ty + 1][tx], sgy[3][ty + 1][tx],
                                 sgy[4][ty + 1][tx]});
+        lane_opaque(own);
         if (own_int) {
-            vjp_output<T, I, Mode>(mu_out, gprim, plane + static_cast<I>(ic + n0 * jc), cell(ty + 1, tx + 1).q,
-                                   weight(ty + 1, tx + 1), gq, dtodv, gm1);
+            vjp_output_at<T, I, Mode>(mu_out, gprim, plane, own, cell(ty + 1, tx + 1).q, weight(ty + 1, tx + 1), gq,
+                                      dtodv, gm1);
         }
 
         carry = gr_z;
         v_cur = v_next;
         w_cur = w_next;
+        h_q = h_qn;
+        h_w = h_wn;
     }
 }
 
