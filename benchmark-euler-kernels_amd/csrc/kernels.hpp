@@ -626,7 +626,7 @@ __device__ __forceinline__ T godunov_tile(
         halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
     }
 
-    T local_min = Limits<T>::max();
+    DtFold<T> local_dt(dt_foldable(invdx0, invdx1, invdx2));
 
     auto stage = [&](int r, int c, Prim<T> const& q, T cs, T e) {
         sq[0][r][c] = q.d;
@@ -745,7 +745,7 @@ __device__ __forceinline__ T godunov_tile(
                 Prim<T> const q1 = cons_to_prim_pt(u, gm1);
                 st_prim(prim_out, plane, own, q1);
                 T const c1 = sound_speed(gamma, q1.d, q1.p);
-                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                local_dt.add(inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
             }
         }
 
@@ -758,7 +758,7 @@ __device__ __forceinline__ T godunov_tile(
         h_cur = h_next;
 #endif
     }
-    return local_min;
+    return local_dt.result();
 }
 
 template <class T, class I, int BY, bool Fused>
@@ -914,7 +914,7 @@ __device__ __forceinline__ T godunov_tile_pair(
         halo = static_cast<uint32_t>(hi + n0 * hj) * static_cast<uint32_t>(sizeof(T));
     }
 
-    T local_min = Limits<T>::max();
+    DtFold<T> local_dt(dt_foldable(invdx0, invdx1, invdx2));
 
     auto stage = [&](int r, int c, Prim<T> const& q, T cs, T e) {
         sq[0][r][c] = q.d;
@@ -1076,7 +1076,7 @@ __device__ __forceinline__ T godunov_tile_pair(
                 st_prim(prim_out, plane, offA, q1);
                 if constexpr (WithDt) {
                     T const c1 = sound_speed(gamma, q1.d, q1.p);
-                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                    local_dt.add(inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
                 }
             }
         }
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ T godunov_tile_pair(
                 st_prim(prim_out, plane, offB, q1);
                 if constexpr (WithDt) {
                     T const c1 = sound_speed(gamma, q1.d, q1.p);
-                    local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                    local_dt.add(inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
                 }
             }
         }
@@ -1103,7 +1103,7 @@ __device__ __forceinline__ T godunov_tile_pair(
         eB = eB_next;
         h_cur = h_next;
     }
-    return local_min;
+    return local_dt.result();
 }
 
 template <class T, class I, int BY, bool Fused>
@@ -1260,7 +1260,7 @@ __device__ __forceinline__ T godunov_tile_lin(
     auto clampa = [&](I x) { return x < s2 - 1 ? x : s2 - 1; };
     uint32_t const own = static_cast<uint32_t>(clampa(a)) * static_cast<uint32_t>(sizeof(T));
 
-    T local_min = Limits<T>::max();
+    DtFold<T> local_dt(dt_foldable(invdx0, invdx1, invdx2));
     auto stage = [&](int r, Prim<T> const& q, T cs, T e) {
         sq[0][r] = q.d;
         sq[1][r] = q.p;
@@ -1364,7 +1364,7 @@ __device__ __forceinline__ T godunov_tile_lin(
                 Prim<T> const q1 = cons_to_prim_pt(u, gm1);
                 st_prim(prim_out, plane, own, q1);
                 T const c1 = sound_speed(gamma, q1.d, q1.p);
-                local_min = min_join(local_min, T(1) / inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
+                local_dt.add(inv_dt_cell(q1, c1, invdx0, invdx1, invdx2));
             }
         }
 
@@ -1373,7 +1373,7 @@ __device__ __forceinline__ T godunov_tile_lin(
         c_cur = c_next;
         e_cur = e_next;
     }
-    return local_min;
+    return local_dt.result();
 }
 
 template <class T, class I, int NW, bool Fused>

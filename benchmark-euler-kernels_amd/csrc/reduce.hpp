@@ -78,6 +78,41 @@ __device__ __forceinline__ T min_join(T incumbent, T candidate)
     return (candidate < incumbent) ? candidate : incumbent;
 }
 
+// Time-step candidates of one thread's cells, folded. A cell's candidate is 1/x,
+// x = sum_d (c + |u_d|) / dx_d, and the step is the Kokkos::Min of the candidates
+// from max(), a NaN candidate never winning (time_step.hpp:28-38). With every
+// 1/dx_d >= 0, x is +0, positive, +inf or NaN; a correctly rounded 1/x never
+// increases as x grows, so the smallest candidate is 1/(largest x), and one
+// division per thread gives the bits of one per cell (x = +0, candidate +inf, and
+// NaN never win either way). With a negative or NaN 1/dx_d (`fold` false, uniform
+// over the launch) each candidate is divided out as before.
+template <class T>
+struct DtFold
+{
+    bool fold;
+    T xmax = T(0);
+    T dtmin = Limits<T>::max();
+
+    __device__ __forceinline__ explicit DtFold(bool f) : fold(f) {}
+
+    __device__ __forceinline__ void add(T x)
+    {
+        if (fold) {
+            xmax = (xmax < x) ? x : xmax;
+        } else {
+            dtmin = min_join(dtmin, T(1) / x);
+        }
+    }
+
+    __device__ __forceinline__ T result() const { return fold ? min_join(dtmin, T(1) / xmax) : dtmin; }
+};
+
+template <class T>
+__device__ __forceinline__ bool dt_foldable(T invdx0, T invdx1, T invdx2)
+{
+    return invdx0 >= T(0) && invdx1 >= T(0) && invdx2 >= T(0);
+}
+
 template <class T>
 __device__ __forceinline__ T wave_min(T v)
 {
