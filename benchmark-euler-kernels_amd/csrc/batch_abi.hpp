@@ -309,7 +309,7 @@ int batch_create(eb_batch** out, inst_c<T> const* inst, int n, int index_bits)
          && hipMemset(b->dt, 0, sizeof(T) * 3 * n) == hipSuccess
          && hipMemset(b->time, 0, sizeof(double) * n) == hipSuccess
          && hipMemset(b->tune_dt, 0, sizeof(T) * n) == hipSuccess;
-    hipLaunchKernelGGL(eb::min_scratch_init, dim3((n + 255) / 256), dim3(256), 0, 0, b->scratch, n);
+    hipLaunchKernelGGL(eb::min_scratch_init, dim3((n * (eb::kMinSlots + 1) + 255) / 256), dim3(256), 0, 0, b->scratch, n);
     ok = ok && hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
         batch_free(b);

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_time_step(
     }
     T const bm = block_min<T, 4>(local);
     if (threadIdx.x == 0) {
-        grid_min_commit(sink, bm, gridDim.x);
+        grid_min_commit(sink, bm, blockIdx.x, gridDim.x);
     }
 }
 
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_time_step_vec(
     }
     T const bm = block_min<T, 4>(local);
     if (threadIdx.x == 0) {
-        grid_min_commit(sink, bm, gridDim.x);
+        grid_min_commit(sink, bm, blockIdx.x, gridDim.x);
     }
 }
 
@@ -795,7 +795,7 @@ __global__ __launch_bounds__(64 * BY) void k_godunov_tiled(
     if constexpr (Fused) {
         T const bm = block_min<T, BY>(local_min);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1140,7 +1140,7 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
     if constexpr (Fused) {
         T const bm = block_min<T, BY / 2>(local_min);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(32 * BY) __attribute__((amdgpu_waves_per_eu(2))) vo
     if constexpr (WithDt) {
         T const bm = block_min<T, BY / 2>(local_min);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1406,7 +1406,7 @@ __global__ __launch_bounds__(64 * NW) void k_godunov_linear(
     if constexpr (Fused) {
         T const bm = block_min<T, NW>(local_min);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1677,7 +1677,7 @@ __global__ __launch_bounds__(64 * (BY + 1)) void k_godunov_stream(
     if constexpr (Fused) {
         T const bm = block_min<T, BY + 1>(local_min);
         if (tx == 0 && ty == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1907,7 +1907,7 @@ __global__ __launch_bounds__(192 * BY) void k_godunov_faces(
     if constexpr (Fused) {
         T const bm = block_min<T, 3 * BY>(local_min);
         if (lid == 0) {
-            grid_min_commit(sink, bm, static_cast<unsigned int>(nblk));
+            grid_min_commit(sink, bm, blockIdx.x, static_cast<unsigned int>(nblk));
         }
     }
 }
@@ -1977,7 +1977,7 @@ __global__ __launch_bounds__(256) void k_ghost_refresh(
     }
     T const bm = block_min<T, 4>(local);
     if (threadIdx.x == 0) {
-        grid_min_commit(sink, bm, gridDim.x);
+        grid_min_commit(sink, bm, blockIdx.x, gridDim.x);
     }
 }
 

@@ -115,8 +115,8 @@ int eb_time_step_sync_f32(eb_context* ctx, hipStream_t stream, const eb_prim_f32
                           float dx0, float dx1, float dx2, float* dt_host);
 /* eb_time_step_f64 without a context, for callers that cannot allocate (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): no
- * reference counterpart. It allocates nothing. Its reduction record (16 bytes: the
- * running minimum and a block count) lives in a static device array that every
+ * reference counterpart. It allocates nothing. Its reduction record (4224 bytes:
+ * 33 accumulators of a running minimum and a count) lives in a static device array that every
  * GPU the library runs on holds until the process ends; the record's size does
  * not grow with the extents, so these are limited only as for eb_time_step_f64.
  * The record is reset by a small kernel at the start of every call. Calls that
