@@ -437,10 +437,20 @@ __device__ __forceinline__ T ld(T const* plane, uint32_t byte_off)
     return *(global_ptr<T const>)(reinterpret_cast<char const*>(plane) + byte_off);
 }
 
+// FP64 stores are non-temporal (streamed past the L2 rather than left dirty in
+// it): a fused step's per-block device-scope release then has little to write
+// back, 128^3 FP64 85 -> 74 us, 512^3 4.89 -> 4.86 ms, the plain flux kernel
+// unchanged. FP32 keeps ordinary stores: non-temporal ones made its fused step
+// 9% slower at 512^3.
 template <class T>
 __device__ __forceinline__ void st(T* plane, uint32_t byte_off, T v)
 {
-    *(global_ptr<T>)(reinterpret_cast<char*>(plane) + byte_off) = v;
+    global_ptr<T> const p = (global_ptr<T>)(reinterpret_cast<char*>(plane) + byte_off);
+    if constexpr (sizeof(T) == 8) {
+        __builtin_nontemporal_store(v, p);
+    } else {
+        *p = v;
+    }
 }
 
 template <class T, class I>
