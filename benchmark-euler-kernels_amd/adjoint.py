@@ -9,10 +9,12 @@ c2p(U_n), exactly as the reference loop does.
 Backward: conservative states are checkpointed every `checkpoint_every` steps;
 each segment is recomputed from its checkpoint (with the recorded dt, so it is
 bitwise the forward) and then reversed step by step through the hand-written HIP
-vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). The last segment's
-primitive states are kept by the forward pass itself. The recompute of the next
-segment runs on a second stream while the current segment's adjoint steps run,
-into a second set of segment buffers. Memory: one state per checkpoint plus two
+vector-Jacobian products (eb_godunov_vjp*, eb_cons_to_prim_vjp). The recompute
+of the next segment runs on a second stream while the current segment's adjoint
+steps run, the two alternating between two sets of segment buffers. The forward
+pass writes the last two segments' primitive states straight into those two sets,
+so the backward starts with both sets holding states (no fill, no recompute for
+them) and needs no checkpoint for them. Memory: one state per checkpoint plus two
 primitive states per step of a segment.
 
 Locality: the adjoint of P_n is zero outside the support box of dL/dP_N grown by
@@ -79,8 +81,10 @@ class Rollout:
         self.prim0: Optional[ops.EulerPrimArrays] = None
         self.prim_final: Optional[ops.EulerPrimArrays] = None
         self._slots: Optional[torch.Tensor] = None  # [0]: discarded dt, [1]: ghost-cell dt minimum
-        self._segment: List[ops.EulerPrimArrays] = []
+        # buffer sets of the last segment (P_tail ...) and the one before (P_kept ...)
+        self._sets: List[List[ops.EulerPrimArrays]] = [[], []]
         self.tail = 0
+        self.kept = 0
         self._tail_ready = False
         self._geom_args = None
         self._ptrs: dict = {}
@@ -145,7 +149,7 @@ class Rollout:
         dtype, dev = prim0.d.dtype, prim0.d.device
         self.prim0 = _clone_prim(prim0)
         self._slots = torch.zeros(2, dtype=dtype, device=dev)
-        self._segment = []
+        self._sets = [[], []]
         cons = _empty_cons_like(prim0)
         ops.prim_to_cons(self.ex, self.prim0, cons, self.eos)
         if frozen_dts is not None:
@@ -157,20 +161,25 @@ class Rollout:
             ops.time_step_async(self.ex, self.prim0, self.eos, self.mesh, out=dt[0:1])
         self.dts = dt[:self.nsteps]
         dt_base, esz = dt.data_ptr(), dt.element_size()
-        # The last segment's states are written straight into the segment buffers,
-        # so the backward starts without recomputing it (and needs no checkpoint for it).
+        # The last two segments' states (P_n, kept <= n < nsteps) are written straight
+        # into the backward's two buffer sets, so the backward recomputes neither (and
+        # needs no checkpoint for them).
         self.tail = ((self.nsteps - 1) // self.k) * self.k if keep else self.nsteps
-        self._ensure_segment(cons, self.nsteps - self.tail, valid=False)
+        self.kept = max(self.tail - self.k, 0) if keep else self.nsteps
+        self._ensure_segment(cons, self.nsteps - self.tail, valid=False, bufs=self._sets[0])
+        self._ensure_segment(cons, self.tail - self.kept, valid=False, bufs=self._sets[1])
         buf = [_clone_prim(prim0), _empty_prim_like(prim0)]
         self._ghost_refresh(cons, buf[1])
         self.checkpoints = {}
         cur = buf[0]
         for n in range(self.nsteps):
-            if keep and n % self.k == 0 and n != self.tail:
+            if keep and n % self.k == 0 and n < self.kept:
                 self.checkpoints[n] = _clone_cons(cons)
-            m = n + 1 - self.tail
-            if 0 <= m < self.nsteps - self.tail:
-                nxt = self._segment[m]
+            p = n + 1  # the step writes P_p
+            if self.tail <= p < self.nsteps:
+                nxt = self._sets[0][p - self.tail]
+            elif self.kept <= p < self.tail:
+                nxt = self._sets[1][p - self.kept]
             else:
                 nxt = buf[1] if cur is buf[0] else buf[0]
             out = self._slot_ptr(0) if frozen_dts is not None else dt_base + (n + 1) * esz
@@ -183,15 +192,14 @@ class Rollout:
         self._tail_ready = keep
         return _clone_prim(self.prim_final)
 
-    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool, bufs=None):
-        """Grows a set of segment buffers (the forward's set by default) to `need`.
+    def _ensure_segment(self, cons: ops.EulerConsArrays, need: int, valid: bool, bufs: list):
+        """Grows a set of segment buffers to `need`.
         valid=True fills new buffers with a finite state (rho = p = 1, u = 0): a
         box-restricted recompute leaves the cells outside its box as they were, and
         the adjoint steps read them with a zero adjoint weight, which needs finite
         values (fills write each field once, where a copy of P_0 would also read it).
         The forward pass writes every interior cell of the buffers it asks for, so
         it passes valid=False."""
-        bufs = self._segment if bufs is None else bufs
         while len(bufs) < need:
             b = _empty_prim_like(cons)
             if valid:
@@ -203,7 +211,7 @@ class Rollout:
             bufs.append(b)
 
     # --------------------------------------------------------------- backward
-    def _recompute(self, start: int, stop: int, needed=None, bufs=None,
+    def _recompute(self, start: int, stop: int, bufs: list, needed=None,
                    ex: Optional[ops.ExecSpace] = None) -> List[ops.EulerPrimArrays]:
         """P_n for n in [start, stop), recomputed from the checkpoint at `start`
         with the recorded time steps (bitwise the forward's states). The backward
@@ -215,10 +223,9 @@ class Rollout:
         step (needed(n+1) grown by one cell lies inside needed(n)); P_n is then
         exact on needed(n) and holds some earlier valid state elsewhere.
 
-        bufs: the buffer set to fill (k valid buffers; the forward's set by
-        default); ex: the execution space (stream) to run on."""
+        bufs: the buffer set to fill (k valid buffers); ex: the execution space
+        (stream) to run on."""
         ex = self.ex if ex is None else ex
-        bufs = self._segment if bufs is None else bufs
         cons = self.checkpoints.pop(start)
         if start == 0:
             prims = [self.prim0]
@@ -240,7 +247,7 @@ class Rollout:
         the checkpoints: one backward per forward."""
         if self.prim_final is None:
             raise RuntimeError("backward before forward")
-        if not self._tail_ready or len(self.checkpoints) != self.tail // self.k:
+        if not self._tail_ready or len(self.checkpoints) != self.kept // self.k:
             raise RuntimeError("the checkpoints of this rollout were already consumed by a backward pass")
         self._tail_ready = False
         suf = self.suf
@@ -282,12 +289,18 @@ class Rollout:
         # Segments last to first. The next segment's states are recomputed on a side
         # stream into the other of two buffer sets while this segment's adjoint
         # steps run on the main stream; events order each set's reuse.
-        segs = list(reversed(sorted(self.checkpoints) + [self.tail]))
-        side = _side_exec(self.ex) if len(segs) > 1 else None
-        sets = [self._segment, []]
-        if len(segs) > 1:
-            any_cons = self.checkpoints[segs[1]]
-            self._ensure_segment(any_cons, self.k, valid=True)
+        # Segment i uses buffer set i % 2: the last segment the forward's set 0, the
+        # one before it set 1 (both written by the forward), the others recomputed.
+        stored = {self.tail: 0}
+        if self.kept < self.tail:
+            stored[self.kept] = 1
+        segs = list(reversed(sorted(self.checkpoints) + sorted(stored)))
+        sets = self._sets
+        side = _side_exec(self.ex) if self.checkpoints else None
+        if self.checkpoints:
+            # ghost conservatives are frozen, so any state gives the buffers' ghosts
+            any_cons = self.checkpoints[max(self.checkpoints)]
+            self._ensure_segment(any_cons, self.k, valid=True, bufs=sets[0])
             self._ensure_segment(any_cons, self.k, valid=True, bufs=sets[1])
         main = self.ex.stream
         if side is not None:
@@ -305,7 +318,7 @@ class Rollout:
             if released[i % 2] is not None:
                 side.stream.wait_event(released[i % 2])
             keep.append(self.checkpoints[start])
-            prims = self._recompute(start, stop, needed, sets[i % 2], side)
+            prims = self._recompute(start, stop, sets[i % 2], needed, side)
             done = torch.cuda.Event()
             done.record(side.stream)
             return prims, done
@@ -313,14 +326,15 @@ class Rollout:
         pending = None
         for i, start in enumerate(segs):
             stop = min(start + self.k, self.nsteps)
-            if i == 0:
-                # stored by the forward pass
-                head = self.prim0 if start == 0 else self._segment[0]
-                prims = [head] + self._segment[1:stop - start]
+            if start in stored:
+                # written by the forward pass
+                s = sets[stored[start]]
+                head = self.prim0 if start == 0 else s[0]
+                prims = [head] + s[1:stop - start]
             else:
                 prims, done = pending
                 main.wait_event(done)
-            pending = recompute_on_side(i + 1) if i + 1 < len(segs) else None
+            pending = recompute_on_side(i + 1) if i + 1 < len(segs) and segs[i + 1] not in stored else None
             for n in range(stop - 1, start - 1, -1):
                 P = prims[n - start]
                 dt = self._dt_ptr(n)
@@ -335,6 +349,8 @@ class Rollout:
             released[i % 2] = torch.cuda.Event()
             released[i % 2].record(main)
         del keep
+        # every side-stream launch has been waited for by the main stream
+        self._sets = [[], []]
         return gprim0
 
 

@@ -407,19 +407,20 @@ def test_localised_rollout_gradient_is_bitwise_checkpoint_invariant(eb, ex, nste
 
 def test_rollout_without_grad_keeps_nothing_and_matches(eb, ex):
     """Under no_grad the rollout stores no checkpoints or states, and its result
-    is bitwise that of the differentiable rollout."""
+    is bitwise that of the differentiable rollout. With gradients, 9 steps and
+    k = 4 keep P_8 and P_4..P_7 (the last two segments) and one checkpoint (step 0)."""
     shape = (18, 14, 12)
     dx = (1.0 / 16, 1.0 / 12, 1.0 / 10)
     P0 = _smooth_state(shape, 3)
     tape = []
     ts, out = _hip_grad(eb, ex, P0, dx, 9, 4, tape=tape)
-    assert len(tape[0].checkpoints) == 2 and len(tape[0]._segment) == 1
+    assert sorted(tape[0].checkpoints) == [0] and [len(b) for b in tape[0]._sets] == [1, 4]
     tape2 = []
     with torch.no_grad():
         prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).cuda() for f in range(5)], index_bits=32)
         out2 = eb.adjoint.rollout(prim, eb.PerfectGas(G), eb.UniformMesh3d(*dx), 0.5, 9, 4, exec_space=ex,
                                   tape=tape2)
-    assert not tape2[0].checkpoints and not tape2[0]._segment
+    assert not tape2[0].checkpoints and not any(tape2[0]._sets)
     for a, b in zip(out, out2):
         assert torch.equal(a.detach(), b)
     with pytest.raises(RuntimeError):
