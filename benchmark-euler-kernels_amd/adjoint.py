@@ -119,9 +119,11 @@ class Rollout:
     def _dt_ptr(self, n: int) -> int:
         return self.dts.data_ptr() + n * self.dts.element_size()
 
-    def _ghost_refresh(self, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays):
+    def _ghost_refresh(self, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, with_dt: bool = True):
+        # with_dt=False: the ghost primitives only (segment buffers; the ghost dt
+        # minimum, slot 1, comes from the first refresh and never changes)
         call(f"eb_ghost_refresh_{self.suf}", self.ex.context, self.ex.handle, self._p(cons), self._p(dst),
-             *self._geom(), self._slot_ptr(1))
+             *self._geom(), self._slot_ptr(1) if with_dt else None)
 
     def _fused(self, src: ops.EulerPrimArrays, cons: ops.EulerConsArrays, dst: ops.EulerPrimArrays, dt_in: int,
                dt_out: Optional[int], region=None, ex: Optional[ops.ExecSpace] = None):
@@ -207,7 +209,7 @@ class Rollout:
                 b.p.fill_(1.0)
                 for u in (b.ux0, b.ux1, b.ux2):
                     u.zero_()
-            self._ghost_refresh(cons, b)  # ghost conservatives are frozen: set once
+            self._ghost_refresh(cons, b, with_dt=False)  # ghost conservatives are frozen: set once
             bufs.append(b)
 
     # --------------------------------------------------------------- backward

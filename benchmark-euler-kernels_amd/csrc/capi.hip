@@ -551,6 +551,15 @@ int run_ghost_refresh(
     int64_t const inner = (n0 > 2 ? n0 - 2 : 0) * (n1 > 2 ? n1 - 2 : 0) * (n2 > 2 ? n2 - 2 : 0);
     auto const c = cons_ptrs<T>(cons);
     eb::ConsPtrs<T const> const cc {c.d, c.e, c.mx0, c.mx1, c.mx2};
+    if (ghost_min == nullptr) {
+        // ghost primitives only: no commit, so one thread per ghost cell
+        int64_t const blocks = std::min<int64_t>(2048, (n - inner + 255) / 256);
+        hipLaunchKernelGGL((eb::k_ghost_refresh<T, I, false>), dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, cc,
+                           prim_ptrs<T>(prim_out), static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2),
+                           gamma, static_cast<T>(gamma - 1), T(1) / dx0, T(1) / dx1, T(1) / dx2,
+                           eb::DtSink<T> {nullptr, nullptr, nullptr, nullptr, nullptr, T(0)});
+        return check_launch("ghost_refresh");
+    }
     hipLaunchKernelGGL((eb::k_ghost_refresh<T, I>), dim3(commit_grid(n - inner)), dim3(256), 0, s, cc,
                        prim_ptrs<T>(prim_out), static_cast<I>(n0), static_cast<I>(n1), static_cast<I>(n2), gamma,
                        static_cast<T>(gamma - 1), T(1) / dx0, T(1) / dx1, T(1) / dx2,
@@ -894,7 +903,7 @@ int abi_ghost_refresh(eb_context* ctx, hipStream_t s, CS const* cons, PS const* 
                       int64_t n2, int bits, T gamma, T dx0, T dx1, T dx2, T* ghost_min)
 {
     EB_CHECK(check_extents(n0, n1, n2, bits));
-    if (ctx == nullptr || ghost_min == nullptr || (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim_out)))) {
+    if (ctx == nullptr || (n0 * n1 * n2 > 0 && (cons_null(cons) || prim_null(prim_out)))) {
         return fail(EB_ERR_INVALID_ARGUMENT, "null argument");
     }
     return bits == 32 ? run_ghost_refresh<T, int32_t>(ctx, s, cons, prim_out, n0, n1, n2, gamma, dx0, dx1, dx2,

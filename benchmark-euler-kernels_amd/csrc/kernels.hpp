@@ -1925,8 +1925,9 @@ __global__ __launch_bounds__(192 * BY) void k_godunov_faces(
 // Ghost-cell refresh for the fused step: prim_out = cons_to_prim(cons) on the
 // ghost layer only (the ghost conservatives never change, godunov.hpp:40-43), and
 // the time-step minimum over those cells published to sink (time_step.hpp:28-38).
-// One thread per ghost cell, enumerated face by face.
-template <class T, class I>
+// One thread per ghost cell, enumerated face by face. Reduce = false writes the
+// ghost primitives only (buffers whose ghost dt is not needed).
+template <class T, class I, bool Reduce = true>
 __global__ __launch_bounds__(256) void k_ghost_refresh(
         ConsPtrs<T const> cons,
         PrimPtrs<T> prim_out,
@@ -1982,12 +1983,16 @@ __global__ __launch_bounds__(256) void k_ghost_refresh(
         I const c = i + n0 * (j + n1 * k);
         Prim<T> const q = cons_to_prim_pt(load_cons(cons, c), gm1);
         store_prim(prim_out, c, q);
-        T const cs = sound_speed(gamma, q.d, q.p);
-        local = min_join(local, T(1) / inv_dt_cell(q, cs, invdx0, invdx1, invdx2));
+        if constexpr (Reduce) {
+            T const cs = sound_speed(gamma, q.d, q.p);
+            local = min_join(local, T(1) / inv_dt_cell(q, cs, invdx0, invdx1, invdx2));
+        }
     }
-    T const bm = block_min<T, 4>(local);
-    if (threadIdx.x == 0) {
-        grid_min_commit(sink, bm, blockIdx.x, gridDim.x);
+    if constexpr (Reduce) {
+        T const bm = block_min<T, 4>(local);
+        if (threadIdx.x == 0) {
+            grid_min_commit(sink, bm, blockIdx.x, gridDim.x);
+        }
     }
 }
 

@@ -155,7 +155,8 @@ int eb_godunov_reference_order_f32(hipStream_t stream, const eb_prim_f32* prim, 
  * One launch performing godunov(prim_in, cons, cfl*dt_in[0]) followed by
  * cons_to_prim(cons -> prim_out) on interior cells and the next time_step over
  * prim_out, whose raw dt is written to dt_out[0]. Ghost cells of prim_out and
- * their dt contribution come from eb_ghost_refresh (ghost_min = its output).
+ * their dt contribution come from eb_ghost_refresh (ghost_min = its output;
+ * with ghost_min_dev NULL it writes the ghost primitives only, no reduction).
  * time_acc (nullable) accumulates cfl*dt_in[0] in FP64. prim_out must differ
  * from prim_in; dt_out must differ from dt_in. */
 int eb_ghost_refresh_f64(eb_context* ctx, hipStream_t stream, const eb_cons_f64* cons, const eb_prim_f64* prim_out,

@@ -471,3 +471,37 @@ def test_step_fused_region_edges(eb, ex, box):
         assert dt_box == -1.0  # nothing launched, dt_out untouched
     assert torch.equal(p_box[:, ~inside], torch.full_like(p_box[:, ~inside], 3.0))
     assert torch.equal(c_box[:, ~inside], torch.stack(cons0.tensors())[:, ~inside])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_ghost_refresh_without_dt_writes_the_same_ghosts(eb, ex, dtype):
+    """eb_ghost_refresh with a null ghost_min (the rollout's segment buffers) writes
+    the same ghost primitives, bitwise, as with one, leaves the interior as it was,
+    and leaves the ghost dt minimum of an earlier refresh untouched."""
+    from benchmark_euler_kernels_amd._lib import call
+
+    ops = eb.operators
+    shape = (21, 17, 13)
+    dx = (1 / 19, 1 / 15, 1 / 11)
+    P0 = _smooth_state(shape, 9)
+    prim = eb.EulerPrimArrays(*[torch.from_numpy(P0[f].copy()).to("cuda", dtype) for f in range(5)])
+    cons = eb.EulerConsArrays(*[torch.empty_like(t) for t in prim.tensors()])
+    ops.prim_to_cons(ex, prim, cons, eb.PerfectGas(G))
+    suf = "f64" if dtype == torch.float64 else "f32"
+    R = ctypes.c_double if dtype == torch.float64 else ctypes.c_float
+    gmin = torch.full((1,), -1.0, dtype=dtype, device="cuda")
+    outs = []
+    for with_dt in (True, False):
+        out = eb.EulerPrimArrays(*[torch.full_like(t, 7.0) for t in prim.tensors()])
+        call(f"eb_ghost_refresh_{suf}", ex.context, ex.handle, ctypes.byref(ops.cons_ptrs(cons)),
+             ctypes.byref(ops.prim_ptrs(out)), *shape, 32, R(G), *map(R, dx), gmin.data_ptr() if with_dt else None)
+        torch.cuda.synchronize()
+        outs.append(torch.stack(out.tensors()))
+    assert torch.equal(outs[0], outs[1])
+    ghost = torch.ones(shape[2], shape[1], shape[0], dtype=torch.bool, device="cuda")
+    ghost[1:-1, 1:-1, 1:-1] = False
+    assert torch.all(outs[1][:, ~ghost] == 7.0)
+    # ghost primitives are cons_to_prim(prim_to_cons(P0)): P0 to rounding
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert torch.allclose(outs[1][:, ghost], torch.stack(prim.tensors())[:, ghost], rtol=tol, atol=tol)
+    assert gmin.item() > 0
