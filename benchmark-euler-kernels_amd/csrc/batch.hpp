@@ -161,7 +161,8 @@ __attribute__((amdgpu_waves_per_eu(Layout == 2 ? 2 : 1))) void k_batch_step(
         if (lid == 0) {
             DtSink<T> const sink {in.scratch, tune ? tune_out + tl.inst : in.dt + (1 - parity), in.dt + 2,
                                   tune ? nullptr : in.time, dt_in, in.cfl};
-            grid_min_commit(sink, bm, static_cast<unsigned int>(tl.local), static_cast<unsigned int>(in.ntiles));
+            // an instance's tiles come off the queue spread out in time
+            grid_min_commit(sink, bm, static_cast<unsigned int>(tl.local), static_cast<unsigned int>(in.ntiles), true);
         }
         __syncthreads(); // closes the commit region inside this iteration
     }

@@ -192,6 +192,40 @@ __device__ __forceinline__ bool slot_commit(MinSlot* s, unsigned long long& enc,
     return true;
 }
 
+// FP32 form of slot_commit with no fence: the running minimum's 32-bit ordered
+// encoding (high half) and the complement of the contribution count (low half)
+// share the accumulator word, updated by compare-and-swap. Every contribution is
+// then a read-modify-write of that one word, whose modifications are totally
+// ordered, so the contributor that brings the count to `expected` holds every
+// contribution's minimum with no release or acquire (a device-scope release
+// writes back the XCD's L2). All-ones is the empty state of both layouts (minimum
+// above every encoding, count 0), so FP32 and FP64 launches can share the words.
+// A failed compare-and-swap costs a round trip, so it suits contributions spread
+// out in time: with the 256 blocks of a one-round launch arriving together, the
+// 32 slot completers queue on the top word (FP32 fused step 64^3 15 -> 24 us),
+// while spread ones gain (batch step 5.26 -> 4.79 ms, 512^3 FP32 fused step
+// 2.73 -> 2.68 ms, time step 0.439 -> 0.419 ms).
+__device__ __forceinline__ bool packed_commit(MinSlot* s, unsigned int& enc, unsigned int expected)
+{
+    unsigned long long seen = 0xffffffffffffffffull; // first guess: empty
+    for (;;) {
+        unsigned int const m = static_cast<unsigned int>(seen >> 32);
+        unsigned int const mn = enc < m ? enc : m;
+        unsigned int const left = static_cast<unsigned int>(seen) - 1u; // ~(count + 1)
+        unsigned long long const next = (static_cast<unsigned long long>(mn) << 32) | left;
+        unsigned long long const was = atomicCAS(&s->acc, seen, next);
+        if (was == seen) {
+            if (~left != expected) {
+                return false;
+            }
+            enc = mn;
+            atomicExch(&s->acc, 0xffffffffffffffffull);
+            return true;
+        }
+        seen = was;
+    }
+}
+
 // Called by one thread of every block with that block's minimum; `rank` numbers
 // the blocks of the reduction 0..nblocks-1. Block r feeds slot r % kMinSlots; the
 // block that completes a slot feeds the top accumulator, and the one that
@@ -199,21 +233,36 @@ __device__ __forceinline__ bool slot_commit(MinSlot* s, unsigned long long& enc,
 // point (~10-20 ns each): with one word for all blocks, the 256 blocks of a 64^3
 // fused step, which arrive together, spent ~6 us there (21.0 -> 18.8 us FP64,
 // 17.4 -> 14.7 us FP32 with the two levels of at most ~nblocks/32 and 32).
+// FP32 takes the fence-free packed form when the blocks arrive spread out: when
+// the caller says so (`spread`, uniform over the launch), or when the launch has
+// more blocks than one per CU (several rounds).
 template <class T>
 __device__ __forceinline__ void grid_min_commit(DtSink<T> const& sink, T block_value, unsigned int rank,
-                                                unsigned int nblocks)
+                                                unsigned int nblocks, bool spread = false)
 {
     unsigned int const slot = rank % kMinSlots;
     unsigned int const in_slot = (nblocks - 1 - slot) / kMinSlots + 1;
     unsigned int const slots = nblocks < kMinSlots ? nblocks : kMinSlots;
-    unsigned long long enc = ord_encode(block_value);
-    if (!slot_commit(&sink.scratch->slot[slot], enc, in_slot)) {
-        return;
+    T m;
+    if (sizeof(T) == 4 && (spread || nblocks > 256)) {
+        unsigned int enc = static_cast<unsigned int>(ord_encode(block_value));
+        if (!packed_commit(&sink.scratch->slot[slot], enc, in_slot)) {
+            return;
+        }
+        if (!packed_commit(&sink.scratch->top, enc, slots)) {
+            return;
+        }
+        m = ord_decode<T>(enc);
+    } else {
+        unsigned long long enc = ord_encode(block_value);
+        if (!slot_commit(&sink.scratch->slot[slot], enc, in_slot)) {
+            return;
+        }
+        if (!slot_commit(&sink.scratch->top, enc, slots)) {
+            return;
+        }
+        m = ord_decode<T>(enc);
     }
-    if (!slot_commit(&sink.scratch->top, enc, slots)) {
-        return;
-    }
-    T m = ord_decode<T>(enc);
     if (sink.extra_min != nullptr) {
         m = min_join(m, *sink.extra_min);
     }
